@@ -1,0 +1,475 @@
+"""CPU oracle: a numpy complex128 restatement of Strawberry Fields' ``fock`` backend.
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker for the HIP library.
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline
+leg import it; the product path (``strawberryfields_amd``) never does.
+
+What it restates (reference files, read as text, never imported):
+
+* gate matrices -- ``strawberryfields/backends/fockbackend/ops.py``:
+  ``phase`` 309-314, ``kerr`` 274-281, ``displacement`` 219-235, ``squeezing``
+  238-254 and ``beamsplitter`` 318-329 (the latter three call
+  ``thewalrus.fock_gradients`` 0.22.0, not vendored; its published algorithm is
+  a ladder-operator recurrence, restated below from the Heisenberg action of
+  each gate), ``lossChannel`` Kraus set 471-490, ``a`` 208-216, ``proj`` 346-353;
+  the loss superoperator of ``tfbackend/ops.py:288-315``.
+* state updates -- ``strawberryfields/backends/fockbackend/circuit.py``:
+  ``apply_gate_BLAS`` 118-217 (the slice loop with ``np.dot``), the
+  selection-rule beamsplitter loop ``_apply_two_mode_passive`` 306-335,
+  ``_apply_channel`` 65-87, ``prepare_multimode`` 393-473, ``norm`` 367-371,
+  ``is_vacuum`` 600-609; ``ops.mix`` 110-120, ``partial_trace`` 144-157.
+* readout -- ``strawberryfields/backends/states.py`` ``BaseFockState``:
+  ``dm`` 544-554, ``trace`` 556-578, ``all_fock_probs`` 580-608,
+  ``reduced_dm`` 613-642, ``fock_prob`` 644-655, ``mean_photon`` 657-663,
+  ``diagonal_expectation`` / ``number_expectation`` 947-980.
+
+Deliberate difference: for a beamsplitter with ``mode1 > mode2`` the fock
+backend's transposes (``circuit.py:236-261``) place the wrong axes in front;
+this oracle (like the TF backend, ``tfbackend/ops.py:800-811``, and the
+``BaseBackend.beamsplitter`` contract) applies the matrix's first mode pair to
+``mode1``.  For ``mode1 < mode2`` both agree.
+
+Pinning (see tests/test_oracle.py): the D, S, BS matrices are checked against
+50-digit mpmath closed forms (tests/golden/gates_mpmath.npz), the BS
+combinatorial prefactors against the reference's own
+``fock_beamsplitter_factors_{5,10}.npz``, the Kraus loss against the
+superoperator identity, the slice-loop gate application against a brute-force
+Kronecker operator, and the restated reference known-answer tests.
+"""
+from itertools import product
+import string
+
+import numpy as np
+
+DEF_TYPE = np.complex128
+_IDX = string.ascii_lowercase
+
+
+# ----------------------------------------------------------------- matrices
+def phase(theta, D):
+    """R(theta) = diag(e^{i n theta})  (fockbackend/ops.py:309-314)."""
+    return np.diag(np.exp(1j * theta * np.arange(D))).astype(DEF_TYPE)
+
+
+def kerr(kappa, D):
+    """K(kappa) = diag(e^{i kappa n^2})  (fockbackend/ops.py:274-281)."""
+    n = np.arange(D)
+    return np.diag(np.exp(1j * kappa * n**2)).astype(DEF_TYPE)
+
+
+def displacement(r, phi, D):
+    """<m|D(r e^{i phi})|n>, truncated, not renormalised.
+
+    Recurrence from D a^dag = (a^dag - alpha*) D and a D = D (a + alpha):
+    D[m,0] = alpha/sqrt(m) D[m-1,0], D[0,0] = e^{-r^2/2},
+    D[m,n] = (sqrt(m) D[m-1,n-1] - alpha* D[m,n-1]) / sqrt(n).
+    """
+    alpha = r * np.exp(1j * phi)
+    U = np.zeros((D, D), dtype=DEF_TYPE)
+    U[0, 0] = np.exp(-0.5 * r**2)
+    for m in range(1, D):
+        U[m, 0] = alpha / np.sqrt(m) * U[m - 1, 0]
+    for n in range(1, D):
+        for m in range(D):
+            v = -np.conj(alpha) * U[m, n - 1]
+            if m > 0:
+                v += np.sqrt(m) * U[m - 1, n - 1]
+            U[m, n] = v / np.sqrt(n)
+    return U
+
+
+def squeezing(r, phi, D):
+    """<m|S(r e^{i phi})|n>, S(z) = exp((z* a^2 - z a^dag^2)/2), truncated.
+
+    Recurrence from S a^dag S^dag = a^dag cosh r + a e^{-i phi} sinh r:
+    S[m,0] = -e^{i phi} tanh r sqrt((m-1)/m) S[m-2,0], S[0,0] = sqrt(sech r),
+    S[m,n] = (sqrt(n-1) e^{-i phi} tanh r S[m,n-2] + sqrt(m) sech r S[m-1,n-1]) / sqrt(n).
+    """
+    th, sech = np.tanh(r), 1.0 / np.cosh(r)
+    e = np.exp(1j * phi)
+    U = np.zeros((D, D), dtype=DEF_TYPE)
+    U[0, 0] = np.sqrt(sech)
+    for m in range(2, D, 2):
+        U[m, 0] = -e * th * np.sqrt((m - 1) / m) * U[m - 2, 0]
+    for n in range(1, D):
+        for m in range(D):
+            if (m + n) % 2:
+                continue
+            v = 0.0j
+            if n >= 2:
+                v += np.sqrt(n - 1) * np.conj(e) * th * U[m, n - 2]
+            if m >= 1:
+                v += np.sqrt(m) * sech * U[m - 1, n - 1]
+            U[m, n] = v / np.sqrt(n)
+    return U
+
+
+def beamsplitter(theta, phi, D):
+    """BS(theta, phi) as mat[out1, in1, out2, in2] (fockbackend/ops.py:318-329 layout).
+
+    BS = exp(theta (e^{i phi} a1 a2^dag - e^{-i phi} a1^dag a2)).  Recurrence from
+    BS a1^dag = (a1^dag cos + e^{i phi} sin a2^dag) BS and
+    BS a2^dag = (a2^dag cos - e^{-i phi} sin a1^dag) BS, photon-number block by block.
+    """
+    ct, st = np.cos(theta), np.sin(theta)
+    e = np.exp(1j * phi)
+    Z = np.zeros((D, D, D, D), dtype=DEF_TYPE)  # Z[N, M, n, m] = <N,M|B|n,m>
+    Z[0, 0, 0, 0] = 1.0
+    for s in range(1, 2 * D - 1):
+        lo, hi = max(0, s - D + 1), min(s, D - 1)
+        for N in range(lo, hi + 1):
+            M = s - N
+            for n in range(lo, hi + 1):
+                m = s - n
+                v = 0.0j
+                if m > 0:
+                    if M >= 1:
+                        v += ct * np.sqrt(M) * Z[N, M - 1, n, m - 1]
+                    if N >= 1:
+                        v -= np.conj(e) * st * np.sqrt(N) * Z[N - 1, M, n, m - 1]
+                    v /= np.sqrt(m)
+                else:
+                    if N >= 1:
+                        v += ct * np.sqrt(N) * Z[N - 1, M, n - 1, 0]
+                    if M >= 1:
+                        v += e * st * np.sqrt(M) * Z[N, M - 1, n - 1, 0]
+                    v /= np.sqrt(n)
+                Z[N, M, n, m] = v
+    return Z.transpose(0, 2, 1, 3).copy()
+
+
+def annihilation(D):
+    """a (fockbackend/ops.py:208-216)."""
+    a = np.zeros((D, D), dtype=DEF_TYPE)
+    for i in range(1, D):
+        a[i - 1, i] = np.sqrt(i)
+    return a
+
+
+def loss_kraus(T, D):
+    """Kraus operators of N(T)  (fockbackend/ops.py:471-490)."""
+    if T == 0:
+        out = []
+        for i in range(D):
+            P = np.zeros((D, D), dtype=DEF_TYPE)
+            P[0, i] = 1.0
+            out.append(P)
+        return out
+    a = annihilation(D)
+    TT = np.diag([T ** (i / 2) for i in range(D)]).astype(DEF_TYPE)
+    ops = []
+    for n in range(D):
+        an = np.linalg.matrix_power(a, n)
+        fac = np.prod(np.arange(1, n + 1, dtype=float))
+        ops.append(((1 - T) / T) ** (n / 2) * (an / np.sqrt(fac)) @ TT)
+    return ops
+
+
+def loss_superop(T, D):
+    """S[a, b, c, d]: |b><c| -> sum S |a><d|  (tfbackend/ops.py:288-315)."""
+    from math import factorial
+
+    S = np.zeros((D, D, D, D))
+    for b in range(D):
+        for c in range(D):
+            for l in range(min(b, c) + 1):
+                coef = (
+                    (1 - T) ** l
+                    * T ** ((b + c) / 2 - l)
+                    * np.sqrt(factorial(b) * factorial(c) / (factorial(b - l) * factorial(c - l)))
+                    / factorial(l)
+                )
+                S[b - l, b, c, c - l] += coef
+    return S
+
+
+# ----------------------------------------------------------------- states
+def mix(state, n):
+    """rho[i0,j0,i1,j1,...] = psi[i...] psi*[j...]  (fockbackend/ops.py:110-120)."""
+    left = [_IDX[i] for i in range(0, 2 * n, 2)]
+    right = [_IDX[i] for i in range(1, 2 * n, 2)]
+    eq = "".join(left) + "," + "".join(right) + "->" + _IDX[: 2 * n]
+    return np.einsum(eq, state, state.conj())
+
+
+def partial_trace(state, n, modes):
+    """Trace out `modes` of a DM  (fockbackend/ops.py:144-157)."""
+    left = [
+        _IDX[2 * i] + _IDX[2 * i] if i in modes else _IDX[2 * i : 2 * i + 2] for i in range(n)
+    ]
+    out = ["" if i in modes else _IDX[2 * i : 2 * i + 2] for i in range(n)]
+    return np.einsum("".join(left) + "->" + "".join(out), state)
+
+
+def vacuum(n, D, pure=True):
+    st = np.zeros([D] * (n if pure else 2 * n), dtype=DEF_TYPE)
+    st.ravel()[0] = 1.0
+    return st
+
+
+class Circuit:
+    """Restatement of fockbackend.circuit.Circuit (the parts on the gate path).
+
+    ``faithful=True`` keeps the reference's loop structure (the slice loop with
+    ``np.dot`` of apply_gate_BLAS and the selection-rule beamsplitter loop); it
+    is the CPU baseline.  ``faithful=False`` contracts with ``tensordot`` /
+    vectorised selection rules and is used to keep the tests fast.  Both are
+    checked against each other and against a brute-force Kronecker operator.
+    """
+
+    def __init__(self, num, trunc, pure=True, faithful=False):
+        if num < 0:
+            raise ValueError("Number of modes must be non-negative -- got {}".format(num))
+        if trunc <= 0:
+            raise ValueError("Truncation must be positive -- got {}".format(trunc))
+        self._num_modes = num
+        self._trunc = trunc
+        self._pure = pure
+        self.faithful = faithful
+        self.reset(pure)
+
+    # -- state management
+    def reset(self, pure=None, cutoff_dim=None, num_subsystems=None):
+        if pure is not None:
+            self._pure = pure
+        if cutoff_dim is not None:
+            self._trunc = cutoff_dim
+        if num_subsystems is not None:
+            self._num_modes = num_subsystems
+        self._state = vacuum(self._num_modes, self._trunc, self._pure)
+
+    def get_state(self):
+        return self._state, self._pure
+
+    # -- generic single-mode gate (circuit.py:118-217)
+    def apply_gate_BLAS(self, mat, modes, state=None, pure=None, n=None):
+        state = self._state if state is None else state
+        pure = self._pure if pure is None else pure
+        n = self._num_modes if n is None else n
+        D = self._trunc
+        size = len(modes)
+        dim = D**size
+        stshape = [D] * size
+        tl = [2 * i for i in range(size)] + [2 * i + 1 for i in range(size)]
+        matview = np.transpose(mat, tl).reshape((dim, dim))
+        diag = np.all(matview == np.diag(np.diagonal(matview)))
+        if pure:
+            if n == 1:
+                return mat.diagonal() * state if diag else mat @ state
+            tlist = [i for i in range(n) if i not in modes] + list(modes)
+            view = np.transpose(state, tlist)
+            if self.faithful:
+                ret = np.zeros([D] * n, dtype=DEF_TYPE)
+                for i in product(*([range(D)] * (n - size))):
+                    if diag:
+                        ret[i] = np.multiply(matview.diagonal(), view[i].ravel()).reshape(stshape)
+                    else:
+                        ret[i] = np.dot(matview, view[i].ravel()).reshape(stshape)
+            else:
+                flat = view.reshape(-1, dim)
+                ret = (flat @ matview.T).reshape([D] * n)
+            return np.transpose(ret, np.argsort(tlist))
+        if n == 1:
+            if diag:
+                d = mat.diagonal().reshape(-1, 1)
+                return d * (state * d.conj().T)
+            return mat @ state @ mat.conj().T
+        tlist = [i for i in range(2 * n) if i // 2 not in modes]
+        tlist = tlist + [2 * i for i in modes] + [2 * i + 1 for i in modes]
+        view = np.transpose(state, tlist)
+        if self.faithful:
+            ret = np.zeros([D] * (2 * n), dtype=DEF_TYPE)
+            for i in product(*([range(D)] * ((n - size) * 2))):
+                if diag:
+                    d = matview.diagonal().reshape(-1, 1)
+                    ret[i] = (d * (view[i].reshape((dim, dim)) * d.conj().T)).reshape(stshape + stshape)
+                else:
+                    ret[i] = (matview @ view[i].reshape((dim, dim)) @ matview.conj().T).reshape(
+                        stshape + stshape
+                    )
+        else:
+            flat = view.reshape(-1, dim, dim)
+            ret = np.matmul(np.matmul(matview, flat), matview.conj().T).reshape([D] * (2 * n))
+        return np.transpose(ret, np.argsort(tlist))
+
+    # -- beamsplitter (circuit.py:219-335), first matrix pair on modes[0]
+    @staticmethod
+    def _two_mode_passive(mat, state, D, faithful):
+        """ret[i, j, ...] = sum_k mat[i, k, j, i+j-k] state[k, i+j-k, ...]  (circuit.py:306-335)."""
+        ret = np.zeros_like(state, dtype=DEF_TYPE)
+        if faithful:
+            for i in range(D):
+                for j in range(D):
+                    for k in range(max(1 + i + j - D, 0), min(i + j, D - 1) + 1):
+                        ret[i, j] += mat[i, k, j, i + j - k] * state[k, i + j - k]
+            return ret
+        for s in range(2 * D - 1):
+            lo, hi = max(0, s - D + 1), min(s, D - 1)
+            ks = np.arange(lo, hi + 1)
+            blk = mat[ks[:, None], ks[None, :], (s - ks)[:, None], (s - ks)[None, :]]
+            sub = state[ks, s - ks]
+            ret[ks, s - ks] = np.tensordot(blk, sub, axes=(1, 0))
+        return ret
+
+    def apply_twomode_gate(self, mat, modes):
+        D, n = self._trunc, self._num_modes
+        st = self._state
+        if self._pure:
+            axes = [modes[0], modes[1]]
+            v = np.moveaxis(st, axes, [0, 1])
+            v = self._two_mode_passive(mat, v, D, self.faithful)
+            return np.moveaxis(v, [0, 1], axes)
+        bra = [2 * modes[0], 2 * modes[1]]
+        ket = [2 * modes[0] + 1, 2 * modes[1] + 1]
+        v = np.moveaxis(st, bra, [0, 1])
+        v = self._two_mode_passive(mat, v, D, self.faithful)
+        v = np.moveaxis(v, [0, 1], bra)
+        v = np.moveaxis(v, ket, [0, 1])
+        v = self._two_mode_passive(mat.conj(), v, D, self.faithful)
+        return np.moveaxis(v, [0, 1], ket)
+
+    # -- channels (circuit.py:65-87)
+    def _apply_channel(self, kraus_ops, modes):
+        if self._pure:
+            self._state = mix(self._state, self._num_modes)
+            self._pure = False
+        states = [self.apply_gate_BLAS(k, modes, pure=False) for k in kraus_ops]
+        self._state = sum(states)
+
+    # -- gates (circuit.py:537-621)
+    def phase_shift(self, theta, mode):
+        self._state = self.apply_gate_BLAS(phase(theta, self._trunc), [mode])
+
+    def kerr_interaction(self, kappa, mode):
+        self._state = self.apply_gate_BLAS(kerr(kappa, self._trunc), [mode])
+
+    def displacement(self, r, phi, mode):
+        self._state = self.apply_gate_BLAS(displacement(r, phi, self._trunc), [mode])
+
+    def squeeze(self, r, phi, mode):
+        self._state = self.apply_gate_BLAS(squeezing(r, phi, self._trunc), [mode])
+
+    def beamsplitter(self, theta, phi, mode1, mode2):
+        self._state = self.apply_twomode_gate(beamsplitter(theta, phi, self._trunc), [mode1, mode2])
+
+    def loss(self, T, mode):
+        self._apply_channel(loss_kraus(T, self._trunc), [mode])
+
+    # -- preparations (circuit.py:393-498), single mode
+    def prepare(self, st, mode):
+        """Replace `mode` by the single-mode ket or DM `st`."""
+        D, n = self._trunc, self._num_modes
+        if n == 1:
+            self._state = np.array(st, dtype=DEF_TYPE)
+            self._pure = st.ndim == 1
+            return
+        if self._pure:
+            self._state = mix(self._state, n)
+            self._pure = False
+        if st.ndim == 1:
+            st = np.outer(st, st.conj())
+        red = partial_trace(self._state, n, [mode])
+        full = np.tensordot(red, st, axes=0)
+        perm = [x for x in range(n) if x != mode] + [mode]
+        index_perm = [2 * x + i for x in perm for i in (0, 1)]
+        self._state = np.transpose(full, np.argsort(index_perm))
+
+    def prepare_mode_fock(self, k, mode):
+        st = np.zeros(self._trunc, dtype=DEF_TYPE)
+        st[k] = 1.0
+        self.prepare(st if self._pure else np.outer(st, st.conj()), mode)
+
+    def norm(self):
+        if self._pure:
+            return np.sqrt(np.vdot(self._state, self._state).real)
+        return np.einsum("".join(_IDX[i] * 2 for i in range(self._num_modes)), self._state)
+
+    def is_vacuum(self, tol):
+        fid = np.abs(self._state.flat[0]) ** 2 if self._pure else self._state.flat[0]
+        return np.abs(fid - 1) <= tol
+
+
+# ----------------------------------------------------------------- readout
+def dm_of(state, n, pure):
+    return mix(state, n) if pure else state
+
+
+def trace(state, n, pure):
+    if pure:
+        return np.vdot(state, state).real
+    return np.einsum("".join(_IDX[i] * 2 for i in range(n)), state).real
+
+
+def all_fock_probs(state, n, pure):
+    D = state.shape[0]
+    if pure:
+        return (np.abs(state) ** 2).reshape([D] * n)
+    ev = list(range(0, 2 * n, 2))
+    od = list(range(1, 2 * n, 2))
+    flat = D**n
+    return np.diag(np.transpose(state, ev + od).reshape(flat, flat)).real.reshape([D] * n)
+
+
+def reduced_dm(state, n, pure, modes):
+    """Reduced DM of sorted `modes`, interleaved axes (states.py:613-642)."""
+    rho = dm_of(state, n, pure)
+    traced = [m for m in range(n) if m not in modes]
+    return partial_trace(rho, n, traced)
+
+
+def fock_prob(state, n, pure, idx):
+    if pure:
+        return np.abs(state[tuple(idx)]) ** 2
+    return state[tuple(idx[i // 2] for i in range(2 * n))].real
+
+
+def diagonal_expectation(state, n, pure, modes, values):
+    """states.py:947-980 (probability-weighted product of per-mode values)."""
+    p = all_fock_probs(state, n, pure)
+    traced = tuple(m for m in range(n) if m not in modes)
+    ps = p.sum(axis=traced) if traced else p
+    # remaining axes are the kept modes in increasing order; weight each
+    kept_sorted = sorted(modes)
+    w = np.ones_like(ps)
+    for ax, _m in enumerate(kept_sorted):
+        shape = [1] * ps.ndim
+        shape[ax] = -1
+        w = w * np.asarray(values).reshape(shape)
+    return float((ps * w).sum())
+
+
+def mean_photon(state, n, pure, mode):
+    D = state.shape[0]
+    r = reduced_dm(state, n, pure, [mode])
+    p = np.diagonal(r).real
+    k = np.arange(D)
+    mean = float((k * p).sum())
+    return mean, float((k**2 * p).sum() - mean**2)
+
+
+# ----------------------------------------------------------------- circuits
+def run_ops(ops, n, D, pure=True, faithful=False, state=None):
+    """Apply a list of (name, modes, params) ops to vacuum (or `state`).
+
+    name in {"R", "K", "D", "S", "BS", "LOSS"}; returns (state, pure).
+    """
+    c = Circuit(n, D, pure=pure, faithful=faithful)
+    if state is not None:
+        c._state = np.array(state, dtype=DEF_TYPE)
+        c._pure = state.ndim == n
+    for name, modes, p in ops:
+        if name == "R":
+            c.phase_shift(p[0], modes[0])
+        elif name == "K":
+            c.kerr_interaction(p[0], modes[0])
+        elif name == "D":
+            c.displacement(p[0], p[1], modes[0])
+        elif name == "S":
+            c.squeeze(p[0], p[1], modes[0])
+        elif name == "BS":
+            c.beamsplitter(p[0], p[1], modes[0], modes[1])
+        elif name == "LOSS":
+            c.loss(p[0], modes[0])
+        else:
+            raise ValueError(name)
+    return c.get_state()

@@ -1,0 +1,298 @@
+"""Minimal mirror of the Strawberry Fields front end for the Fock gate path.
+
+Same names and behaviour as the reference for the operations this backend
+runs: ``Program`` (``strawberryfields/program.py``), ``Engine`` /
+``LocalEngine`` (``engine.py:48-572``), ``Result`` (``result.py``), free
+parameters (``parameters.py``: ``prog.params`` / ``eng.run(args=...)``) and the
+operations ``Sgate``, ``Dgate``, ``Rgate``, ``Kgate``, ``BSgate``,
+``LossChannel``, ``Vacuum``, ``Fock``, ``Coherent``, ``Squeezed``
+(``ops.py:541-1948``).  ``Gate.apply`` keeps the reference rules: a gate whose
+first parameter is zero is skipped and a daggered gate negates its first
+parameter (``ops.py:485-509``).  With Strawberry Fields installed, use
+``sf.Engine(backend=HIPFockBackend())`` instead; this module exists so the
+backend can be driven the same way without it.
+"""
+import numpy as np
+import torch
+
+_current = []
+
+
+class RegRef:
+    def __init__(self, ind):
+        self.ind = ind
+
+    def __repr__(self):
+        return "q[{}]".format(self.ind)
+
+
+class FreeParameter:
+    """Named placeholder bound at run time (parameters.py:FreeParameter)."""
+
+    def __init__(self, name):
+        self.name = name
+        self.val = None
+
+    def __neg__(self):
+        return _Expr(lambda v: -v, self)
+
+    def __repr__(self):
+        return "{" + self.name + "}"
+
+
+class _Expr:
+    def __init__(self, fn, par):
+        self.fn, self.par = fn, par
+
+
+def par_evaluate(p):
+    """Value of a gate parameter: numbers pass through, free parameters resolve."""
+    if isinstance(p, FreeParameter):
+        if p.val is None:
+            raise ValueError("free parameter {} is not bound".format(p.name))
+        return p.val
+    if isinstance(p, _Expr):
+        return p.fn(par_evaluate(p.par))
+    return p
+
+
+class Command:
+    def __init__(self, op, reg):
+        self.op, self.reg = op, reg
+
+
+class Program:
+    """Quantum circuit on ``num_subsystems`` modes (program.py:77-901, the used subset)."""
+
+    def __init__(self, num_subsystems):
+        self.num_subsystems = num_subsystems
+        self.register = tuple(RegRef(i) for i in range(num_subsystems))
+        self.circuit = []
+        self.free_params = {}
+
+    @property
+    def context(self):
+        return self
+
+    def __enter__(self):
+        _current.append(self)
+        return self.register
+
+    def __exit__(self, *exc):
+        _current.pop()
+
+    def append(self, op, reg):
+        self.circuit.append(Command(op, reg))
+        return reg
+
+    def params(self, *names):
+        out = []
+        for n in names:
+            if n not in self.free_params:
+                self.free_params[n] = FreeParameter(n)
+            out.append(self.free_params[n])
+        return out[0] if len(out) == 1 else out
+
+    def bind_params(self, binding):
+        for k, v in binding.items():
+            if k not in self.free_params:
+                raise ValueError("unknown free parameter {}".format(k))
+            self.free_params[k].val = v
+
+
+class Operation:
+    ns = None
+
+    def __init__(self, par):
+        self.p = list(par)
+
+    def __or__(self, reg):
+        if isinstance(reg, RegRef):
+            reg = [reg]
+        reg = list(reg)
+        if self.ns is not None and self.ns != len(reg):
+            raise ValueError("Wrong number of subsystems.")
+        if not _current:
+            raise RuntimeError("operations must be applied inside a Program context")
+        return _current[-1].append(self, reg)
+
+    def apply(self, reg, backend, **kwargs):
+        self._apply([r.ind for r in reg], backend, **kwargs)
+
+
+class Gate(Operation):
+    def __init__(self, par):
+        super().__init__(par)
+        self.dagger = False
+
+    @property
+    def H(self):
+        import copy
+
+        g = copy.copy(self)
+        g.dagger = not self.dagger
+        return g
+
+    def apply(self, reg, backend, **kwargs):
+        z = par_evaluate(self.p[0])
+        if not isinstance(z, torch.Tensor) and np.all(np.asarray(z) == 0):
+            return
+        if self.dagger:
+            z = -z
+        orig = self.p[0]
+        self.p[0] = z
+        try:
+            self._apply([r.ind for r in reg], backend, **kwargs)
+        finally:
+            self.p[0] = orig
+
+    def _vals(self):
+        return [par_evaluate(x) for x in self.p]
+
+
+class Dgate(Gate):
+    ns = 1
+
+    def __init__(self, r, phi=0.0):
+        super().__init__([r, phi])
+
+    def _apply(self, reg, backend, **kwargs):
+        r, phi = self._vals()
+        backend.displacement(r, phi, *reg)
+
+
+class Sgate(Gate):
+    ns = 1
+
+    def __init__(self, r, phi=0.0):
+        super().__init__([r, phi])
+
+    def _apply(self, reg, backend, **kwargs):
+        r, phi = self._vals()
+        backend.squeeze(r, phi, *reg)
+
+
+class Rgate(Gate):
+    ns = 1
+
+    def __init__(self, theta):
+        super().__init__([theta])
+
+    def _apply(self, reg, backend, **kwargs):
+        (theta,) = self._vals()
+        backend.rotation(theta, *reg)
+
+
+class Kgate(Gate):
+    ns = 1
+
+    def __init__(self, kappa):
+        super().__init__([kappa])
+
+    def _apply(self, reg, backend, **kwargs):
+        (kappa,) = self._vals()
+        backend.kerr_interaction(kappa, *reg)
+
+
+class BSgate(Gate):
+    ns = 2
+
+    def __init__(self, theta=np.pi / 4, phi=0.0):
+        super().__init__([theta, phi])
+
+    def _apply(self, reg, backend, **kwargs):
+        theta, phi = self._vals()
+        backend.beamsplitter(theta, phi, *reg)
+
+
+class LossChannel(Operation):
+    ns = 1
+
+    def __init__(self, T):
+        super().__init__([T])
+
+    def _apply(self, reg, backend, **kwargs):
+        backend.loss(par_evaluate(self.p[0]), *reg)
+
+
+class Vacuum(Operation):
+    ns = 1
+
+    def __init__(self):
+        super().__init__([])
+
+    def _apply(self, reg, backend, **kwargs):
+        backend.prepare_vacuum_state(*reg)
+
+
+class Fock(Operation):
+    ns = 1
+
+    def __init__(self, n=0):
+        super().__init__([n])
+
+    def _apply(self, reg, backend, **kwargs):
+        backend.prepare_fock_state(int(par_evaluate(self.p[0])), *reg)
+
+
+class Coherent(Operation):
+    ns = 1
+
+    def __init__(self, r=0.0, phi=0.0):
+        super().__init__([r, phi])
+
+    def _apply(self, reg, backend, **kwargs):
+        backend.prepare_coherent_state(par_evaluate(self.p[0]), par_evaluate(self.p[1]), *reg)
+
+
+class Squeezed(Operation):
+    ns = 1
+
+    def __init__(self, r=0.0, p=0.0):
+        super().__init__([r, p])
+
+    def _apply(self, reg, backend, **kwargs):
+        backend.prepare_squeezed_state(par_evaluate(self.p[0]), par_evaluate(self.p[1]), *reg)
+
+
+class Result:
+    """Outcome of Engine.run (result.py): the final state."""
+
+    def __init__(self, state):
+        self.state = state
+        self.samples = np.empty((0, 0))
+
+
+class LocalEngine:
+    """Engine.run semantics of engine.py:229-572 for a backend instance."""
+
+    def __init__(self, backend, *, backend_options=None):
+        self.backend = backend
+        self.backend_name = backend.short_name
+        self.backend_options = dict(backend_options or {})
+        self._initialized = False
+        self.run_progs = []
+
+    def reset(self, backend_options=None):
+        if backend_options is not None:
+            self.backend_options.update(backend_options)
+        if self._initialized:
+            self.backend.reset(**self.backend_options)
+        self.run_progs = []
+
+    def run(self, program, *, args=None, compile_options=None, **kwargs):
+        if kwargs.get("shots", 1) not in (1, None):
+            raise NotImplementedError("only shots=1 is supported")
+        if not self._initialized:
+            self.backend.begin_circuit(num_subsystems=program.num_subsystems, **self.backend_options)
+            self._initialized = True
+        if args:
+            program.bind_params(args)
+        for cmd in program.circuit:
+            cmd.op.apply(cmd.reg, self.backend)
+        self.run_progs.append(program)
+        modes = kwargs.get("modes", None)
+        return Result(self.backend.state(modes=modes))
+
+
+Engine = LocalEngine

@@ -1,0 +1,422 @@
+"""Device runtime: planned circuits, the differentiable circuit Function, readout.
+
+A circuit is a list of ``(name, modes)`` operations with one row of two real
+parameters per operation.  ``CompiledCircuit`` plans it once on the library
+(``sfamd_prog_create``), owns the device workspaces, and runs
+
+* the gate-table build (all matrices of the circuit, FP64 recurrences, one
+  launch per 96 operations) from a device parameter vector,
+* the forward sweep, writing requested operation inputs as checkpoints,
+* the adjoint sweep, which returns the state cotangent and every parameter
+  gradient.
+
+``CircuitFunction`` wraps this in ``torch.autograd.Function``: the backward
+recomputes each checkpoint segment forward and runs the hand-written adjoint
+kernels over it (truncated Fock gates are not unitary, so the reverse sweep
+needs the forward states; it cannot un-compute them).
+
+This replaces the TF backend's tape (``tfbackend/circuit.py:72-76`` keeps every
+state; ``tfbackend/ops.py:190-204,318-366`` give the per-gate VJPs).
+"""
+import ctypes
+import math
+
+import numpy as np
+import torch
+
+from . import _lib
+
+# default memory budget for keeping every operation input alive in backward
+DEFAULT_SAVE_BUDGET = 16 * 2**30
+
+
+def _elem_bytes(dtype):
+    return 8 if dtype == torch.complex64 else 16
+
+
+class CompiledCircuit:
+    """A planned circuit (structure only) bound to one device and dtype."""
+
+    def __init__(self, ops, n, D, mixed_in, dtype, device):
+        self.lib = _lib.load()
+        self.ops = [(name, tuple(int(m) for m in modes)) for name, modes in ops]
+        self.n, self.D, self.mixed_in = int(n), int(D), bool(mixed_in)
+        self.dtype = dtype
+        self.code = _lib.dtype_code(dtype)
+        self.device = torch.device(device)
+        arr = (_lib.OpDesc * max(1, len(self.ops)))()
+        for i, (name, modes) in enumerate(self.ops):
+            arr[i].kind = _lib.OP_KINDS[name]
+            arr[i].mode0 = modes[0] if len(modes) > 0 else -1
+            arr[i].mode1 = modes[1] if len(modes) > 1 else -1
+            arr[i].pad = 0
+        handle = ctypes.c_void_p()
+        st = self.lib.sfamd_prog_create(
+            arr, len(self.ops), self.n, self.D, int(self.mixed_in), self.code, ctypes.byref(handle)
+        )
+        _lib.check(st, "sfamd_prog_create")
+        self.handle = handle
+        info = _lib.ProgInfo()
+        _lib.check(self.lib.sfamd_prog_info(handle, ctypes.byref(info)), "sfamd_prog_info")
+        self.info = info
+        self.nops = len(self.ops)
+        self.final_mixed = bool(info.final_mixed)
+        self.state_elems = [self.lib.sfamd_prog_state_elems(handle, i) for i in range(self.nops + 1)]
+        self.mixed_at = []
+        m = self.mixed_in
+        for name, _ in self.ops:
+            self.mixed_at.append(m)
+            if name in ("LOSS", "MIX"):
+                m = True
+        self.mixed_at.append(m)
+        dev = self.device
+        self.table = torch.empty(max(1, info.table_elems), dtype=dtype, device=dev)
+        self.ws = torch.empty(max(1, info.ws_elems), dtype=dtype, device=dev)
+        self.partial = torch.empty(max(1, info.partial_doubles), dtype=torch.float64, device=dev)
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            try:
+                self.lib.sfamd_prog_destroy(h)
+            except Exception:  # interpreter shutdown
+                pass
+            self.handle = None
+
+    # ---- shapes
+    def state_shape(self, i):
+        return [self.D] * (2 * self.n if self.mixed_at[i] else self.n)
+
+    def state_bytes(self, i):
+        return self.state_elems[i] * _elem_bytes(self.dtype)
+
+    def new_state(self, i):
+        return torch.empty(self.state_shape(i), dtype=self.dtype, device=self.device)
+
+    # ---- launches
+    def _stream(self):
+        return _lib.stream_ptr(self.device)
+
+    def build_table(self, params):
+        """params: float64 [nops, 2] on the device."""
+        assert params.dtype == torch.float64 and params.is_contiguous()
+        if self.nops == 0:
+            return
+        _lib.call(
+            "sfamd_prog_build_table", self.handle, _lib.ptr(params), _lib.ptr(self.table), self._stream()
+        )
+
+    def forward(self, begin, end, state_in, out=None, saved=None):
+        """Run ops [begin, end) on state_in; saved[j] receives the input of op begin+j."""
+        if out is None:
+            out = self.new_state(end)
+        sv = None
+        if saved is not None:
+            sv = (ctypes.c_void_p * max(1, end - begin))()
+            for j, t in enumerate(saved):
+                sv[j] = t.data_ptr() if t is not None else None
+        _lib.call(
+            "sfamd_prog_forward",
+            self.handle,
+            begin,
+            end,
+            _lib.ptr(self.table),
+            _lib.ptr(state_in),
+            _lib.ptr(out),
+            sv,
+            _lib.ptr(self.ws),
+            self._stream(),
+        )
+        return out
+
+    def adjoint(self, begin, end, saved, g_out, g_in, grads):
+        sv = (ctypes.c_void_p * max(1, end - begin))()
+        for j, t in enumerate(saved):
+            sv[j] = t.data_ptr()
+        _lib.call(
+            "sfamd_prog_adjoint",
+            self.handle,
+            begin,
+            end,
+            _lib.ptr(self.table),
+            sv,
+            _lib.ptr(g_out),
+            _lib.ptr(g_in),
+            _lib.ptr(grads),
+            _lib.ptr(self.partial),
+            _lib.ptr(self.ws),
+            self._stream(),
+        )
+
+    # ---- checkpoint plan
+    def segments(self, segment=None, budget=DEFAULT_SAVE_BUDGET):
+        """Split [0, nops) into checkpoint segments.
+
+        segment=None keeps every operation input when they fit in `budget`
+        bytes, else uses segments of ~sqrt(nops) operations; an int gives the
+        segment length (e.g. the layer length of a layered circuit).
+        """
+        if self.nops == 0:
+            return []
+        if segment is None:
+            total = sum(self.state_bytes(i) for i in range(self.nops))
+            if total <= budget:
+                segment = self.nops
+            else:
+                segment = max(1, int(math.ceil(math.sqrt(self.nops))))
+        segment = max(1, int(segment))
+        return [(b, min(b + segment, self.nops)) for b in range(0, self.nops, segment)]
+
+
+def _params_row(p):
+    """Two float64 parameter slots (tensor, keeps autograd) of one operation."""
+    out = []
+    for v in list(p)[:2]:
+        if isinstance(v, torch.Tensor):
+            out.append(v.reshape(()).to(torch.float64))
+        else:
+            out.append(torch.tensor(float(v), dtype=torch.float64))
+    while len(out) < 2:
+        out.append(torch.tensor(0.0, dtype=torch.float64))
+    return out
+
+
+def params_tensor(ops_params, device):
+    """Stack per-op parameters into a float64 [nops, 2] device tensor (differentiable)."""
+    rows = []
+    for p in ops_params:
+        rows.extend(_params_row(p))
+    if not rows:
+        return torch.zeros((0, 2), dtype=torch.float64, device=device)
+    flat = torch.stack([r.cpu() if r.device.type != "cpu" else r for r in rows])
+    return flat.reshape(-1, 2).to(device)
+
+
+class CircuitFunction(torch.autograd.Function):
+    """state_out = circuit(params, state_in), with the hand-written adjoint as backward."""
+
+    @staticmethod
+    def forward(ctx, params, state_in, cc, segment):
+        params = params.detach().contiguous()
+        state_in = state_in.detach().contiguous()
+        cc.build_table(params)
+        segs = cc.segments(segment)
+        ckpts = []
+        cur = state_in
+        for (b, e) in segs:
+            ckpts.append(cur)
+            cur = cc.forward(b, e, cur)
+        ctx.cc = cc
+        ctx.segs = segs
+        ctx.params = params
+        ctx.save_for_backward(*ckpts) if ckpts else None
+        ctx.n_ckpt = len(ckpts)
+        return cur if segs else state_in.clone()
+
+    @staticmethod
+    def backward(ctx, g_out):
+        cc = ctx.cc
+        if not ctx.segs:
+            return None, g_out, None, None
+        ckpts = ctx.saved_tensors
+        # the gate table may have been rebuilt by another run of the same plan
+        cc.build_table(ctx.params)
+        grads = torch.zeros(cc.nops * 2, dtype=torch.float64, device=cc.device)
+        lam = g_out.to(cc.dtype).contiguous()
+        for si in range(len(ctx.segs) - 1, -1, -1):
+            b, e = ctx.segs[si]
+            saved = [ckpts[si]] + [cc.new_state(i) for i in range(b + 1, e)]
+            if e - b > 1:
+                cc.forward(b, e - 1, ckpts[si], out=saved[-1], saved=saved[:-1])
+            g_in = cc.new_state(b)
+            cc.adjoint(b, e, saved, lam, g_in, grads)
+            lam = g_in
+        return grads.reshape(-1, 2), lam, None, None
+
+
+def run_circuit(cc, params, state_in, segment=None):
+    """Differentiable execution when params or state_in require grad, plain otherwise."""
+    if torch.is_grad_enabled() and (params.requires_grad or state_in.requires_grad):
+        return CircuitFunction.apply(params, state_in, cc, segment)
+    params = params.detach().contiguous()
+    cc.build_table(params)
+    if cc.nops == 0:
+        return state_in.clone()
+    return cc.forward(0, cc.nops, state_in.detach().contiguous())
+
+
+# --------------------------------------------------------------------- readout
+def _scratch(n, dtype, device):
+    return torch.empty(max(1, n), dtype=dtype, device=device)
+
+
+def norm2(state, n, D, mixed):
+    """<psi|psi> or Tr rho, FP64 accumulation; returns a 0-dim float64 device tensor."""
+    lib = _lib.load()
+    dev = state.device
+    part = _scratch(lib.sfamd_reduce_partials(n, D, int(mixed)), torch.float64, dev)
+    out = torch.empty(1, dtype=torch.float64, device=dev)
+    _lib.call(
+        "sfamd_norm",
+        _lib.ptr(state),
+        n,
+        D,
+        int(mixed),
+        _lib.ptr(part),
+        _lib.ptr(out),
+        _lib.dtype_code(state.dtype),
+        _lib.stream_ptr(dev),
+    )
+    return out[0]
+
+
+class _ProbsFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, state, n, D, mixed):
+        state = state.detach().contiguous()
+        probs = torch.empty([D] * n, dtype=_lib.real_dtype(state.dtype), device=state.device)
+        _lib.call(
+            "sfamd_all_fock_probs",
+            _lib.ptr(state),
+            _lib.ptr(probs),
+            n,
+            D,
+            int(mixed),
+            _lib.dtype_code(state.dtype),
+            _lib.stream_ptr(state.device),
+        )
+        ctx.save_for_backward(state)
+        ctx.n, ctx.D, ctx.mixed = n, D, mixed
+        return probs
+
+    @staticmethod
+    def backward(ctx, g):
+        (state,) = ctx.saved_tensors
+        g = g.to(_lib.real_dtype(state.dtype)).contiguous()
+        gs = torch.empty_like(state)
+        _lib.call(
+            "sfamd_all_fock_probs_vjp",
+            _lib.ptr(state),
+            _lib.ptr(g),
+            _lib.ptr(gs),
+            ctx.n,
+            ctx.D,
+            int(ctx.mixed),
+            _lib.dtype_code(state.dtype),
+            _lib.stream_ptr(state.device),
+        )
+        return gs, None, None, None
+
+
+def all_fock_probs(state, n, D, mixed):
+    return _ProbsFunction.apply(state, n, D, mixed)
+
+
+def reduced_dm(state, n, D, mixed, modes):
+    """Reduced DM of sorted `modes` ([D]*2k, interleaved) straight from the state."""
+    lib = _lib.load()
+    k = len(modes)
+    dev = state.device
+    part = _scratch(lib.sfamd_reduced_dm_partials(n, D, k, int(mixed)), torch.float64, dev)
+    out = torch.empty([D] * (2 * k), dtype=state.dtype, device=dev)
+    arr = (ctypes.c_int * max(1, k))(*modes)
+    _lib.call(
+        "sfamd_reduced_dm",
+        _lib.ptr(state.contiguous()),
+        _lib.ptr(out),
+        n,
+        D,
+        int(mixed),
+        arr,
+        k,
+        _lib.ptr(part),
+        _lib.dtype_code(state.dtype),
+        _lib.stream_ptr(dev),
+    )
+    return out
+
+
+def diagonal_expectation(state, n, D, mixed, modes, values):
+    """sum_I p(I) prod_{m in modes} values[I_m] (FP64); returns a 0-dim float64 tensor."""
+    lib = _lib.load()
+    dev = state.device
+    k = len(modes)
+    part = _scratch(lib.sfamd_reduce_partials(n, D, int(mixed)), torch.float64, dev)
+    out = torch.empty(1, dtype=torch.float64, device=dev)
+    vals = torch.as_tensor(np.asarray(values, dtype=np.float64)).to(dev)
+    arr = (ctypes.c_int * max(1, k))(*modes)
+    _lib.call(
+        "sfamd_diagonal_expectation",
+        _lib.ptr(state.contiguous()),
+        n,
+        D,
+        int(mixed),
+        arr,
+        k,
+        _lib.ptr(vals),
+        _lib.ptr(part),
+        _lib.ptr(out),
+        _lib.dtype_code(state.dtype),
+        _lib.stream_ptr(dev),
+    )
+    return out[0]
+
+
+def mix(ket, n, D):
+    dm = torch.empty([D] * (2 * n), dtype=ket.dtype, device=ket.device)
+    _lib.call(
+        "sfamd_mix",
+        _lib.ptr(ket.contiguous()),
+        _lib.ptr(dm),
+        n,
+        D,
+        _lib.dtype_code(ket.dtype),
+        _lib.stream_ptr(ket.device),
+    )
+    return dm
+
+
+def replace_mode(dm, n, D, mode, sigma):
+    """rho' = sigma (x) Tr_mode(rho) with sigma a [D, D] device matrix."""
+    out = torch.empty_like(dm)
+    _lib.call(
+        "sfamd_replace_mode",
+        _lib.ptr(dm.contiguous()),
+        _lib.ptr(out),
+        n,
+        D,
+        mode,
+        _lib.ptr(sigma.to(dm.dtype).contiguous()),
+        _lib.dtype_code(dm.dtype),
+        _lib.stream_ptr(dm.device),
+    )
+    return out
+
+
+def build_gate(kind, p0, p1, D, dtype, device):
+    """Gate-table entry of one gate (matrix + derivatives) as a flat device tensor."""
+    lib = _lib.load()
+    code = _lib.OP_KINDS[kind]
+    size = lib.sfamd_gate_table_size(code, D)
+    if size < 0:
+        raise ValueError("cutoff %d not supported for gate %s" % (D, kind))
+    out = torch.empty(max(1, size), dtype=dtype, device=device)
+    _lib.call(
+        "sfamd_build_gate",
+        code,
+        float(p0),
+        float(p1),
+        D,
+        _lib.dtype_code(dtype),
+        _lib.ptr(out),
+        _lib.stream_ptr(torch.device(device)),
+    )
+    return out
+
+
+def gate_matrix(kind, p0, p1, D, dtype, device):
+    """Dense [D, D] matrix of a single-mode gate (R and K as diagonal matrices)."""
+    t = build_gate(kind, p0, p1, D, dtype, device)
+    if kind in ("R", "K"):
+        return torch.diag(t[:D])
+    return t[: D * D].reshape(D, D)

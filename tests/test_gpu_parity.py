@@ -1,0 +1,380 @@
+"""GPU parity: the HIP path against the CPU oracle on the same seeded inputs.
+
+Tolerances: complex128 max|d psi| <= 1e-12 (1e-11 for long circuits),
+complex64 <= 2e-5; gradients against central finite differences of the
+complex128 oracle.
+"""
+import itertools
+import os
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import fock_oracle as fo
+from strawberryfields_amd import HIPFockBackend, circuits, frontend, runtime
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+C128, C64 = torch.complex128, torch.complex64
+TOL = {C128: 1e-12, C64: 2e-5}
+DEV = "cuda:0"
+
+
+def apply(be, name, modes, p):
+    if name == "R":
+        be.rotation(p[0], modes[0])
+    elif name == "K":
+        be.kerr_interaction(p[0], modes[0])
+    elif name == "D":
+        be.displacement(p[0], p[1], modes[0])
+    elif name == "S":
+        be.squeeze(p[0], p[1], modes[0])
+    elif name == "BS":
+        be.beamsplitter(p[0], p[1], modes[0], modes[1])
+    elif name == "LOSS":
+        be.loss(p[0], modes[0])
+    else:
+        raise ValueError(name)
+
+
+def run_backend(ops, n, D, dtype, pure=True, init=None, **kw):
+    be = HIPFockBackend()
+    be.begin_circuit(n, cutoff_dim=D, pure=pure, dtype=dtype, device=DEV, **kw)
+    if init is not None:
+        if init.ndim == n:
+            be.prepare_ket_state(init, list(range(n)))
+        else:
+            be.prepare_dm_state(init, list(range(n)))
+    for name, modes, p in ops:
+        apply(be, name, modes, p)
+    return be
+
+
+def to_np(t):
+    return t.detach().cpu().numpy()
+
+
+def rand_state(n, D, pure, seed):
+    rng = np.random.default_rng(seed)
+    psi = rng.normal(size=[D] * n) + 1j * rng.normal(size=[D] * n)
+    psi /= np.linalg.norm(psi)
+    return psi if pure else fo.mix(psi, n) * 0.7 + 0.3 * fo.mix(np.roll(psi, 1), n)
+
+
+# ------------------------------------------------------------------ gate tables
+@pytest.mark.parametrize("dtype", [C128, C64])
+@pytest.mark.parametrize("D", [1, 2, 5, 12, 20])
+def test_single_mode_gate_tables(dtype, D):
+    tol = 1e-13 if dtype == C128 else 1e-6
+    for kind, fn, params in [
+        ("D", fo.displacement, [(0.3, 0.7), (-0.45, 2.1), (0.9, -1.3)]),
+        ("S", fo.squeezing, [(0.3, 0.7), (-0.4, 2.1), (0.8, -1.3)]),
+    ]:
+        for p0, p1 in params:
+            t = to_np(runtime.build_gate(kind, p0, p1, D, dtype, DEV))
+            DD = D * D
+            U = t[:DD].reshape(D, D)
+            assert np.abs(U - fn(p0, p1, D)).max() < tol, (kind, p0, p1)
+            h = 1e-6
+            d0 = (fn(p0 + h, p1, D) - fn(p0 - h, p1, D)) / (2 * h)
+            d1 = (fn(p0, p1 + h, D) - fn(p0, p1 - h, D)) / (2 * h)
+            dtol = 1e-8 if dtype == C128 else 1e-5
+            assert np.abs(t[DD : 2 * DD].reshape(D, D) - d0).max() < dtol, (kind, "d0")
+            assert np.abs(t[2 * DD :].reshape(D, D) - d1).max() < dtol, (kind, "d1")
+    for kind, fn in [("R", fo.phase), ("K", fo.kerr)]:
+        t = to_np(runtime.build_gate(kind, 0.37, 0.0, D, dtype, DEV))
+        assert np.abs(t[:D] - np.diag(fn(0.37, D))).max() < tol
+
+
+def _bs_dense_from_blocks(blk, D):
+    mat = np.zeros((D,) * 4, dtype=complex)
+    off = 0
+    for s in range(2 * D - 1):
+        lo, hi = max(0, s - D + 1), min(s, D - 1)
+        c = hi - lo + 1
+        b = blk[off : off + c * c].reshape(c, c)
+        for a in range(c):
+            for k in range(c):
+                N, n = lo + a, lo + k
+                mat[N, n, s - N, s - n] = b[a, k]
+        off += c * c
+    return mat, off
+
+
+@pytest.mark.parametrize("dtype", [C128, C64])
+@pytest.mark.parametrize("D", [1, 2, 6, 15, 20])
+def test_beamsplitter_and_loss_tables(dtype, D):
+    tol = 1e-13 if dtype == C128 else 1e-6
+    for th, ph in [(0.6, 0.4), (-0.35, 1.7), (1.4, -2.2)]:
+        t = to_np(runtime.build_gate("BS", th, ph, D, dtype, DEV))
+        B, nb = _bs_dense_from_blocks(t, D)
+        assert np.abs(B - fo.beamsplitter(th, ph, D)).max() < tol
+        h = 1e-6
+        dB = (fo.beamsplitter(th + h, ph, D) - fo.beamsplitter(th - h, ph, D)) / (2 * h)
+        dBg, _ = _bs_dense_from_blocks(t[nb:], D)
+        assert np.abs(dBg - dB).max() < (1e-8 if dtype == C128 else 1e-5)
+    for T in [0.0, 0.42, 1.0]:
+        t = runtime.build_gate("LOSS", T, 0.0, D, dtype, DEV)
+        c = to_np(torch.view_as_real(t).reshape(-1)[: D**3]).reshape(D, D, D)
+        S = fo.loss_superop(T, D)  # S[p, p+l, q+l, q]
+        ref = np.zeros((D, D, D))
+        for p, q, l in itertools.product(range(D), repeat=3):
+            if p + l < D and q + l < D:
+                ref[p, q, l] = S[p, p + l, q + l, q]
+        assert np.abs(c - ref).max() < tol * 10
+
+
+# ------------------------------------------------------------------ single operations
+SINGLE = [
+    ("R", (0.7,)),
+    ("K", (-0.3,)),
+    ("D", (0.4, 1.2)),
+    ("D", (-0.25, -0.6)),
+    ("S", (0.3, -0.8)),
+    ("S", (-0.2, 0.5)),
+    ("LOSS", (0.63,)),
+]
+
+
+@pytest.mark.parametrize("dtype", [C128, C64])
+@pytest.mark.parametrize("pure", [True, False])
+@pytest.mark.parametrize("n,D", [(1, 1), (1, 6), (2, 2), (3, 5)])
+def test_single_mode_ops(dtype, pure, n, D):
+    init = rand_state(n, D, pure, seed=n * 100 + D)
+    for (name, p), mode in itertools.product(SINGLE, range(n)):
+        ops = [(name, (mode,), p)]
+        be = run_backend(ops, n, D, dtype, pure=pure, init=init)
+        st = be.state()
+        ref, rpure = fo.run_ops(ops, n, D, state=init)
+        assert st.is_pure == rpure
+        got = to_np(st.ket() if rpure else st.dm())
+        assert np.abs(got - ref).max() < TOL[dtype] * max(1, np.abs(ref).max()), (name, p, mode)
+
+
+@pytest.mark.parametrize("dtype", [C128, C64])
+@pytest.mark.parametrize("pure", [True, False])
+@pytest.mark.parametrize("n,D", [(2, 1), (2, 4), (3, 5), (4, 3)])
+def test_beamsplitter_all_mode_pairs(dtype, pure, n, D):
+    init = rand_state(n, D, pure, seed=7 + n + D)
+    for m1, m2 in itertools.permutations(range(n), 2):
+        for p in [(0.7, 0.9), (-1.1, 2.4)]:
+            ops = [("BS", (m1, m2), p)]
+            be = run_backend(ops, n, D, dtype, pure=pure, init=init)
+            ref, rpure = fo.run_ops(ops, n, D, state=init)
+            st = be.state()
+            got = to_np(st.ket() if rpure else st.dm())
+            assert np.abs(got - ref).max() < TOL[dtype], (m1, m2, p)
+
+
+def test_loss_extremes_and_vacuum():
+    D = 8
+    for T in [0.0, 1.0]:
+        be = run_backend([("D", (0,), (0.6, 0.3)), ("LOSS", (0,), (T,))], 1, D, C128)
+        ref, _ = fo.run_ops([("D", (0,), (0.6, 0.3)), ("LOSS", (0,), (T,))], 1, D)
+        assert np.abs(to_np(be.state().dm()) - ref).max() < 1e-13
+    be = run_backend([("D", (0,), (0.6, 0.3)), ("LOSS", (0,), (0.0,))], 1, D, C128)
+    assert be.is_vacuum(1e-12)
+    be = run_backend([("LOSS", (0,), (0.5,))], 1, D, C128)
+    assert be.is_vacuum(1e-14)
+
+
+# ------------------------------------------------------------------ circuits
+def _fidelity(a, b):
+    a, b = a.reshape(-1), b.reshape(-1)
+    return abs(np.vdot(a, b)) ** 2 / (np.vdot(a, a).real * np.vdot(b, b).real)
+
+
+@pytest.mark.parametrize("dtype", [C128, C64])
+def test_config1_through_engine_matches_golden(dtype):
+    c1 = circuits.config1()
+    g = np.load(os.path.join(GOLDEN, "c1.npz"))
+    prog = frontend.Program(3)
+    gates = {"S": frontend.Sgate, "D": frontend.Dgate, "R": frontend.Rgate, "K": frontend.Kgate, "BS": frontend.BSgate}
+    with prog.context as q:
+        for name, modes, p in c1["ops"]:
+            gates[name](*p) | tuple(q[m] for m in modes)
+    eng = frontend.Engine(HIPFockBackend(), backend_options={"cutoff_dim": 8, "dtype": dtype, "device": DEV})
+    state = eng.run(prog).state
+    ket = to_np(state.ket())
+    assert np.abs(ket - g["ket"]).max() < TOL[dtype]
+    assert 1 - _fidelity(ket, g["ket"]) < (1e-13 if dtype == C128 else 1e-6)
+    probs = to_np(state.all_fock_probs())
+    assert np.abs(probs - g["probs"]).max() < TOL[dtype]
+
+
+@pytest.mark.parametrize("dtype", [C128, C64])
+def test_config2_forward(dtype):
+    c2 = circuits.config2()
+    ref, _ = fo.run_ops(c2["ops"], 4, 15)
+    be = run_backend(c2["ops"], 4, 15, dtype)
+    st = be.state()
+    ket = to_np(st.ket())
+    tol = 1e-11 if dtype == C128 else 2e-5
+    assert np.abs(ket - ref).max() < tol
+    assert abs(st.trace() - fo.trace(ref, 4, True)) < tol
+    for idx in [(0, 0, 0, 0), (1, 0, 2, 1), (3, 3, 0, 1)]:
+        assert abs(st.fock_prob(list(idx)) - fo.fock_prob(ref, 4, True, idx)) < tol
+
+
+def test_config3_forward_complex64():
+    c3 = circuits.config3()
+    ref, _ = fo.run_ops(c3["ops"], 6, 8)
+    st = run_backend(c3["ops"], 6, 8, C64).state()
+    ket = to_np(st.ket())
+    assert np.abs(ket - ref).max() < 2e-5
+    assert 1 - _fidelity(ket, ref) < 1e-6
+
+
+def test_mixed_layer_with_loss_matches_oracle():
+    """One layer of the checkpointed mixed-state workload at reduced cutoff."""
+    rng = np.random.default_rng(5)
+    ops = circuits.layered(3, 1, rng, loss=True)
+    for dtype in (C128, C64):
+        be = run_backend(ops, 3, 6, dtype, pure=False)
+        ref, _ = fo.run_ops(ops, 3, 6, pure=False)
+        assert np.abs(to_np(be.state().dm()) - ref).max() < TOL[dtype] * 10
+
+
+# ------------------------------------------------------------------ readout
+@pytest.mark.parametrize("pure", [True, False])
+def test_readouts(pure):
+    n, D = 4, 4
+    rng = np.random.default_rng(9)
+    ops = circuits.layered(n, 1, rng)
+    init = None if pure else fo.vacuum(n, D, pure=False)
+    be = run_backend(ops, n, D, C128, pure=pure)
+    ref, rpure = fo.run_ops(ops, n, D, pure=pure)
+    st = be.state()
+    assert abs(st.trace() - fo.trace(ref, n, rpure)) < 1e-13
+    assert np.abs(to_np(st.all_fock_probs()) - fo.all_fock_probs(ref, n, rpure)).max() < 1e-14
+    for modes in ([0], [3], [1, 2], [0, 2, 3]):
+        assert np.abs(to_np(st.reduced_dm(modes)) - fo.reduced_dm(ref, n, rpure, modes)).max() < 1e-13
+    for m in range(n):
+        mean, var = st.mean_photon(m)
+        rm, rv = fo.mean_photon(ref, n, rpure, m)
+        assert abs(mean - rm) < 1e-12 and abs(var - rv) < 1e-12
+    vals = np.linspace(-1, 2, D)
+    for modes in ([1], [0, 3], [2, 1, 0]):
+        assert abs(st.diagonal_expectation(modes, vals) - fo.diagonal_expectation(ref, n, rpure, modes, vals)) < 1e-12
+    # reduced state through backend.state(modes) in requested order
+    sub = be.state(modes=[2, 0])
+    r = fo.reduced_dm(ref, n, rpure, [0, 2]).transpose(2, 3, 0, 1)
+    assert np.abs(to_np(sub.dm()) - r).max() < 1e-13
+    del init
+
+
+def test_prepare_vacuum_on_pure_multimode_state_becomes_mixed():
+    n, D = 3, 5
+    ops = [("D", (0,), (0.5, 0.2)), ("BS", (0, 1), (0.7, 0.1)), ("S", (2,), (0.3, 0.0))]
+    be = run_backend(ops, n, D, C128)
+    be.prepare_vacuum_state(1)
+    st = be.state()
+    assert not st.is_pure
+    c = fo.Circuit(n, D)
+    for name, modes, p in ops:
+        getattr(c, {"D": "displacement", "BS": "beamsplitter", "S": "squeeze"}[name])(*p, *modes)
+    c.prepare_mode_fock(0, 1)
+    assert np.abs(to_np(st.dm()) - c._state).max() < 1e-13
+
+
+# ------------------------------------------------------------------ gradients
+def _loss_pure(state, target):
+    ov = torch.vdot(target.reshape(-1), state.reshape(-1))
+    return 1 - ov.abs() ** 2
+
+
+def _run_params(ops, n, D, dtype, pure, params, segment=None):
+    be = HIPFockBackend()
+    be.begin_circuit(n, cutoff_dim=D, pure=pure, dtype=dtype, device=DEV, checkpoint_segment=segment)
+    k = 0
+    for name, modes, p in ops:
+        vals = [params[k + j] for j in range(len(p))]
+        k += len(p)
+        apply(be, name, modes, vals)
+    return be.state()
+
+
+def _oracle_loss(ops, n, D, pure, flat, target, weights):
+    k = 0
+    new = []
+    for name, modes, p in ops:
+        new.append((name, modes, tuple(flat[k : k + len(p)])))
+        k += len(p)
+    st, rp = fo.run_ops(new, n, D, pure=pure)
+    if target is not None:
+        return 1 - abs(np.vdot(target.reshape(-1), st.reshape(-1))) ** 2
+    return float((fo.all_fock_probs(st, n, rp) * weights).sum())
+
+
+def _fd_grad(ops, n, D, pure, target, weights, h=1e-5):
+    flat = np.array([v for _, _, p in ops for v in p], dtype=float)
+    g = np.zeros_like(flat)
+    for i in range(len(flat)):
+        a, b = flat.copy(), flat.copy()
+        a[i] += h
+        b[i] -= h
+        g[i] = (_oracle_loss(ops, n, D, pure, a, target, weights) - _oracle_loss(ops, n, D, pure, b, target, weights)) / (2 * h)
+    return flat, g
+
+
+@pytest.mark.parametrize("dtype", [C128, C64])
+def test_gradient_pure_layer_vs_finite_differences(dtype):
+    n, D = 3, 6
+    rng = np.random.default_rng(21)
+    ops = circuits.layered(n, 1, rng, sd=0.3)
+    target = circuits.target_state(n, D, 4, rng)
+    flat, g_fd = _fd_grad(ops, n, D, True, target, None)
+    params = torch.tensor(flat, dtype=torch.float64, requires_grad=True)
+    st = _run_params(ops, n, D, dtype, True, params)
+    loss = _loss_pure(st.ket(), torch.as_tensor(target, dtype=dtype, device=DEV))
+    loss.backward()
+    g = params.grad.numpy()
+    rel = np.linalg.norm(g - g_fd) / np.linalg.norm(g_fd)
+    assert rel < (1e-8 if dtype == C128 else 1e-5), rel
+
+
+@pytest.mark.parametrize("dtype", [C128, C64])
+def test_gradient_mixed_with_loss_vs_finite_differences(dtype):
+    n, D = 2, 6
+    rng = np.random.default_rng(22)
+    ops = circuits.layered(n, 1, rng, sd=0.3, loss=True)
+    idx = np.indices([D] * n).sum(axis=0)
+    weights = (idx - n) ** 2.0
+    for pure in (True, False):  # pure start: the first loss converts the ket
+        flat, g_fd = _fd_grad(ops, n, D, pure, None, weights)
+        params = torch.tensor(flat, dtype=torch.float64, requires_grad=True)
+        st = _run_params(ops, n, D, dtype, pure, params)
+        loss = (st.all_fock_probs().to(torch.float64) * torch.as_tensor(weights, device=DEV)).sum()
+        loss.backward()
+        g = params.grad.numpy()
+        rel = np.linalg.norm(g - g_fd) / np.linalg.norm(g_fd)
+        assert rel < (1e-8 if dtype == C128 else 1e-5), (pure, rel)
+
+
+def test_checkpoint_segments_give_identical_gradients():
+    n, D = 3, 5
+    rng = np.random.default_rng(23)
+    ops = circuits.layered(n, 2, rng, loss=True)
+    flat = np.array([v for _, _, p in ops for v in p], dtype=float)
+    grads = []
+    for seg in (None, 1, 7, 24):
+        params = torch.tensor(flat, dtype=torch.float64, requires_grad=True)
+        st = _run_params(ops, n, D, C64, False, params, segment=seg)
+        st.all_fock_probs().sum().backward()
+        grads.append(params.grad.numpy().copy())
+    for g in grads[1:]:
+        assert np.array_equal(g, grads[0])
+
+
+def test_mean_photon_gradient_closed_form():
+    """test_tf_integration.py: d<n>/dr of a coherent state is 2r."""
+    r = torch.tensor(0.3, dtype=torch.float64, requires_grad=True)
+    be = HIPFockBackend()
+    be.begin_circuit(1, cutoff_dim=25, dtype=C128, device=DEV)
+    be.displacement(r, 0.4, 0)
+    mean, _ = be.state().mean_photon(0)
+    mean.backward()
+    assert abs(float(mean) - 0.09) < 1e-12
+    assert abs(float(r.grad) - 0.6) < 1e-10

@@ -1,0 +1,182 @@
+"""CPU-only checks of the library boundary and the host logic (no GPU calls).
+
+* libsfamd.so loads and exports every function include/sfamd.h declares;
+* host-side planning and validation return the documented status codes;
+* the front-end mirror keeps the reference's Gate.apply rules;
+* the circuit generators produce the documented workloads.
+"""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from strawberryfields_amd import _lib, circuits, frontend
+from strawberryfields_amd.base import ModeMap
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    txt = open(os.path.join(ROOT, "include", "sfamd.h")).read()
+    return sorted(set(re.findall(r"\b(sfamd_\w+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = _declared()
+    assert len(names) >= 25
+    for name in names:
+        assert hasattr(lib, name), name
+        assert name in _lib.SIGNATURES, name
+
+
+def test_status_strings_and_version():
+    lib = _lib.load()
+    assert lib.sfamd_version() >= 1
+    assert lib.sfamd_status_string(0) == b"success"
+    assert b"cutoff" in lib.sfamd_status_string(-2)
+
+
+def test_gate_table_sizes():
+    lib = _lib.load()
+    D = 7
+    assert lib.sfamd_gate_table_size(_lib.OP_ROTATION, D) == D
+    assert lib.sfamd_gate_table_size(_lib.OP_DISPLACEMENT, D) == 3 * D * D
+    nb = sum((min(s, D - 1) - max(0, s - D + 1) + 1) ** 2 for s in range(2 * D - 1))
+    assert nb == D * (2 * D * D + 1) // 3
+    assert lib.sfamd_gate_table_size(_lib.OP_BEAMSPLITTER, D) == 2 * nb
+    assert lib.sfamd_gate_table_size(_lib.OP_LOSS, D) == D**3
+    assert lib.sfamd_gate_table_size(_lib.OP_BEAMSPLITTER, 41) == -1
+    assert lib.sfamd_gate_table_size(_lib.OP_SQUEEZE, 65) == -1
+
+
+def _create(ops, n, D, mixed=0, dtype=_lib.C64):
+    lib = _lib.load()
+    arr = (_lib.OpDesc * max(1, len(ops)))()
+    for i, (k, a, b) in enumerate(ops):
+        arr[i].kind, arr[i].mode0, arr[i].mode1 = k, a, b
+    h = ctypes.c_void_p()
+    st = lib.sfamd_prog_create(arr, len(ops), n, D, mixed, dtype, ctypes.byref(h))
+    return st, h
+
+
+def test_program_planning_and_validation():
+    lib = _lib.load()
+    D, n = 5, 3
+    ops = [
+        (_lib.OP_DISPLACEMENT, 0, -1),
+        (_lib.OP_BEAMSPLITTER, 2, 0),
+        (_lib.OP_LOSS, 1, -1),
+        (_lib.OP_KERR, 2, -1),
+    ]
+    st, h = _create(ops, n, D)
+    assert st == 0
+    info = _lib.ProgInfo()
+    assert lib.sfamd_prog_info(h, ctypes.byref(info)) == 0
+    assert info.nops == 4 and info.final_mixed == 1
+    assert info.final_elems == D ** (2 * n)
+    assert lib.sfamd_prog_state_elems(h, 0) == D**n
+    assert lib.sfamd_prog_state_elems(h, 2) == D**n
+    assert lib.sfamd_prog_state_elems(h, 3) == D ** (2 * n)
+    assert info.ws_elems == 4 * D ** (2 * n)
+    expect = 3 * D * D + D * (2 * D * D + 1) // 3 * 2 + D**3 + D
+    assert info.table_elems == expect
+    assert lib.sfamd_prog_destroy(h) == 0
+    # invalid programs are rejected on the host with the documented codes
+    assert _create([(_lib.OP_BEAMSPLITTER, 1, 1)], n, D)[0] == -1
+    assert _create([(_lib.OP_ROTATION, 3, -1)], n, D)[0] == -1
+    assert _create([(_lib.OP_ROTATION, 0, -1)], n, 65)[0] == -2
+    assert _create([(_lib.OP_BEAMSPLITTER, 0, 1)], n, 41)[0] == -2
+    assert _create([(9, 0, -1)], n, D)[0] == -3
+    assert _create([(_lib.OP_MIX, -1, -1)], n, D, mixed=1)[0] == -4
+    assert _create([], n, D, dtype=7)[0] == -1
+
+
+def test_null_pointer_arguments_rejected_without_gpu_work():
+    lib = _lib.load()
+    assert lib.sfamd_apply_1mode(_lib.OP_ROTATION, None, None, None, 2, 4, 0, 0, None, 0, None) == -1
+    assert lib.sfamd_norm(None, 2, 4, 0, None, None, 0, None) == -1
+    assert lib.sfamd_graph_begin(None) == -1
+
+
+# ------------------------------------------------------------------ front end
+class RecordingBackend:
+    short_name = "rec"
+
+    def __init__(self):
+        self.calls = []
+
+    def begin_circuit(self, num_subsystems, **kw):
+        self.calls.append(("begin", num_subsystems, kw))
+
+    def __getattr__(self, name):
+        def f(*args):
+            self.calls.append((name,) + args)
+
+        return f
+
+    def state(self, modes=None):
+        return "state"
+
+
+def test_gate_apply_rules():
+    prog = frontend.Program(2)
+    with prog.context as q:
+        frontend.Dgate(0.0, 0.3) | q[0]  # skipped: first parameter zero
+        frontend.Dgate(0.2, 0.3).H | q[0]  # dagger negates r
+        frontend.Sgate(0.1) | q[1]
+        frontend.BSgate(0.4, 0.1) | (q[1], q[0])
+        frontend.LossChannel(0.9) | q[0]
+    be = RecordingBackend()
+    eng = frontend.Engine(be, backend_options={"cutoff_dim": 4})
+    res = eng.run(prog)
+    assert res.state == "state"
+    assert be.calls[0] == ("begin", 2, {"cutoff_dim": 4})
+    assert be.calls[1] == ("displacement", -0.2, 0.3, 0)
+    assert be.calls[2] == ("squeeze", 0.1, 0.0, 1)
+    assert be.calls[3] == ("beamsplitter", 0.4, 0.1, 1, 0)
+    assert be.calls[4] == ("loss", 0.9, 0)
+
+
+def test_free_parameters_bind_at_run():
+    prog = frontend.Program(1)
+    a = prog.params("a")
+    with prog.context as q:
+        frontend.Rgate(a) | q[0]
+    be = RecordingBackend()
+    frontend.Engine(be, backend_options={"cutoff_dim": 3}).run(prog, args={"a": 0.25})
+    assert be.calls[-1] == ("rotation", 0.25, 0)
+
+
+def test_modemap_delete_and_add():
+    mm = ModeMap(4)
+    mm.delete(1)
+    assert mm.show() == [0, None, 1, 2]
+    mm.add(1)
+    assert mm.show() == [0, None, 1, 2, 3]
+    assert not mm.valid([5])
+    with pytest.raises(ValueError):
+        mm.delete(7)
+
+
+# ------------------------------------------------------------------ workloads
+def test_workload_generators():
+    c1 = circuits.config1()
+    assert len(c1["ops"]) == 15 and c1["n"] == 3 and c1["D"] == 8
+    c2 = circuits.config2()
+    assert len(c2["ops"]) == 256
+    c3 = circuits.config3()
+    assert len(c3["ops"]) == 240 and circuits.n_params(c3["ops"]) == 408
+    t = c3["target"].reshape(-1)
+    assert abs(np.vdot(t, t) - 1) < 1e-12
+    idx = np.indices([8] * 6).reshape(6, -1).sum(0)
+    assert np.all(t[idx > 6] == 0)
+    c4 = circuits.config4()
+    assert len(c4["circuits"]) == 256
+    assert all(c["D"] ** c["n"] <= 2**20 for c in c4["circuits"])
+    c5 = circuits.config5()
+    assert len(c5["ops"]) == 144 and sum(1 for o in c5["ops"] if o[0] == "LOSS") == 18
+    # same seed -> same parameters
+    assert circuits.config3()["ops"] == c3["ops"]

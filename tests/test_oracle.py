@@ -1,0 +1,325 @@
+"""Pins the CPU oracle (oracle/fock_oracle.py) before it is trusted as the checker.
+
+Each check is chosen so that a plausible slip (a dropped term, a wrong sign,
+a transposed operand, a swapped mode) fails it:
+  * gate matrices vs 50-digit mpmath closed forms (tests/golden/gates_mpmath.npz);
+  * BS combinatorial factors vs the reference's own data files;
+  * Kraus loss vs the independent superoperator identity;
+  * slice-loop gate application vs a brute-force Kronecker operator;
+  * the reference's known-answer tests, restated at tight tolerances.
+"""
+import itertools
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+from scipy.special import comb, factorial
+
+from oracle import fock_oracle as fo
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ------------------------------------------------------------------ matrices
+def test_gate_matrices_match_mpmath_closed_forms():
+    g = np.load(os.path.join(GOLDEN, "gates_mpmath.npz"))
+    fns = {"D": fo.displacement, "S": fo.squeezing, "BS": fo.beamsplitter}
+    n = 0
+    for k in g.files:
+        if k.endswith("_params"):
+            continue
+        p0, p1, D = g[k + "_params"]
+        m = fns[k.split("_")[0]](p0, p1, int(D))
+        assert np.abs(m - g[k]).max() < 1e-13, k
+        n += 1
+    assert n == 11
+
+
+@pytest.mark.parametrize("D", [5, 10])
+def test_beamsplitter_factors_match_reference_data(D):
+    """fock_beamsplitter_factors_D.npz (reference data) holds prefac[N,n,M,m,k]."""
+    ref = sp.load_npz(os.path.join(GOLDEN, "fock_beamsplitter_factors_%d.npz" % D)).toarray()
+    ref = ref.reshape(D, D, D, D, D).real
+    P = np.zeros((D,) * 5)
+    for N, n, M, m in itertools.product(range(D), repeat=4):
+        if N + M != n + m:
+            continue
+        for k in range(D):
+            v = comb(n, k) * comb(m, N - k) * comb(N, k) * comb(M, n - k)
+            if v > 0:
+                P[N, n, M, m, k] = (-1) ** (N - k) * np.sqrt(v)
+    assert np.abs(ref - P).max() == 0.0
+    # the oracle's recurrence reproduces the closed form built on the data
+    for theta, phi in [(0.3, 1.1), (1.2, -0.4), (-0.7, 2.5)]:
+        B = fo.beamsplitter(theta, phi, D)
+        ct, st = np.cos(theta), np.sin(theta)
+        C = np.zeros((D,) * 4, dtype=complex)
+        for N, n, M, m in itertools.product(range(D), repeat=4):
+            if N + M != n + m:
+                continue
+            s = sum(
+                ref[N, n, M, m, k] * ct ** (m - N + 2 * k) * st ** (n + N - 2 * k)
+                for k in range(D)
+                if ref[N, n, M, m, k] != 0
+            )
+            C[N, n, M, m] = np.exp(1j * phi * (n - N)) * s
+        assert np.abs(B - C).max() < 1e-12
+
+
+def test_beamsplitter_blocks_unitary_on_complete_blocks():
+    D = 12
+    B = fo.beamsplitter(0.83, 0.41, D)
+    for s in range(D):
+        ks = np.arange(0, s + 1)
+        blk = B[ks[:, None], ks[None, :], (s - ks)[:, None], (s - ks)[None, :]]
+        assert np.abs(blk @ blk.conj().T - np.eye(s + 1)).max() < 1e-13
+
+
+def test_rotation_kerr_exact():
+    D = 9
+    assert np.allclose(np.diag(fo.phase(0.37, D)), np.exp(1j * 0.37 * np.arange(D)), atol=0, rtol=1e-15)
+    assert np.allclose(np.diag(fo.kerr(0.2, D)), np.exp(1j * 0.2 * np.arange(D) ** 2), atol=0, rtol=1e-15)
+
+
+@pytest.mark.parametrize("T", [0.0, 0.3, 0.77, 1.0])
+def test_loss_kraus_equals_superoperator(T):
+    D = 7
+    rng = np.random.default_rng(3)
+    rho = rng.normal(size=(D, D)) + 1j * rng.normal(size=(D, D))
+    kr = sum(E @ rho @ E.conj().T for E in fo.loss_kraus(T, D))
+    S = fo.loss_superop(T, D)
+    sup = np.einsum("abcd,bc->ad", S, rho)
+    assert np.abs(kr - sup).max() < 1e-13
+
+
+# ------------------------------------------------------------------ application
+def _kron_op(mats_by_mode, n, D):
+    op = np.array([[1.0 + 0j]])
+    for m in range(n):
+        op = np.kron(op, mats_by_mode.get(m, np.eye(D)))
+    return op
+
+
+def _bs_full(mat, m1, m2, n, D):
+    """Dense D^n operator of a beamsplitter with mat's first pair on m1."""
+    dim = D**n
+    op = np.zeros((dim, dim), dtype=complex)
+    for I in range(dim):
+        di = np.unravel_index(I, [D] * n)
+        for J in range(dim):
+            dj = np.unravel_index(J, [D] * n)
+            if any(di[k] != dj[k] for k in range(n) if k not in (m1, m2)):
+                continue
+            op[I, J] = mat[di[m1], dj[m1], di[m2], dj[m2]]
+    return op
+
+
+@pytest.mark.parametrize("faithful", [True, False])
+def test_single_mode_gates_vs_kronecker(faithful):
+    rng = np.random.default_rng(11)
+    n, D = 3, 4
+    psi = rng.normal(size=[D] * n) + 1j * rng.normal(size=[D] * n)
+    rho = fo.mix(psi, n)
+    mats = {
+        "R": fo.phase(0.7, D),
+        "K": fo.kerr(0.3, D),
+        "D": fo.displacement(0.4, 1.2, D),
+        "S": fo.squeezing(0.3, -0.8, D),
+    }
+    for name, U in mats.items():
+        for mode in range(n):
+            c = fo.Circuit(n, D, pure=True, faithful=faithful)
+            c._state = psi.copy()
+            out = c.apply_gate_BLAS(U, [mode])
+            ref = (_kron_op({mode: U}, n, D) @ psi.reshape(-1)).reshape([D] * n)
+            assert np.abs(out - ref).max() < 1e-12, (name, mode)
+            c2 = fo.Circuit(n, D, pure=False, faithful=faithful)
+            c2._state = rho.copy()
+            outm = c2.apply_gate_BLAS(U, [mode])
+            K = _kron_op({mode: U}, n, D)
+            refm = fo.mix((K @ psi.reshape(-1)).reshape([D] * n), n)
+            assert np.abs(outm - refm).max() < 1e-10, (name, mode, "mixed")
+
+
+@pytest.mark.parametrize("faithful", [True, False])
+def test_beamsplitter_vs_kronecker_both_orders(faithful):
+    rng = np.random.default_rng(12)
+    n, D = 3, 4
+    psi = rng.normal(size=[D] * n) + 1j * rng.normal(size=[D] * n)
+    mat = fo.beamsplitter(0.7, 0.9, D)
+    for m1, m2 in itertools.permutations(range(n), 2):
+        c = fo.Circuit(n, D, pure=True, faithful=faithful)
+        c._state = psi.copy()
+        c.beamsplitter(0.7, 0.9, m1, m2)
+        ref = (_bs_full(mat, m1, m2, n, D) @ psi.reshape(-1)).reshape([D] * n)
+        assert np.abs(c._state - ref).max() < 1e-12, (m1, m2)
+        cm = fo.Circuit(n, D, pure=False, faithful=faithful)
+        cm._state = fo.mix(psi, n)
+        cm.beamsplitter(0.7, 0.9, m1, m2)
+        assert np.abs(cm._state - fo.mix(ref, n)).max() < 1e-10, (m1, m2, "mixed")
+
+
+def test_faithful_and_fast_agree_on_config1():
+    from strawberryfields_amd import circuits
+
+    c1 = circuits.config1()
+    a, _ = fo.run_ops(c1["ops"], 3, 8, faithful=True)
+    b, _ = fo.run_ops(c1["ops"], 3, 8, faithful=False)
+    assert np.abs(a - b).max() < 1e-14
+    g = np.load(os.path.join(GOLDEN, "c1.npz"))
+    assert np.abs(a - g["ket"]).max() < 1e-14
+
+
+def test_loss_on_pure_then_mixed_matches_kraus_by_hand():
+    n, D = 2, 5
+    ops = [("D", (0,), (0.3, 0.2)), ("LOSS", (0,), (0.6,)), ("S", (1,), (0.2, 0.5)), ("LOSS", (1,), (0.8,))]
+    st, pure = fo.run_ops(ops, n, D)
+    assert not pure
+    psi = fo.vacuum(n, D)
+    K0 = _kron_op({0: fo.displacement(0.3, 0.2, D)}, n, D)
+    v = K0 @ psi.reshape(-1)
+    rho = np.outer(v, v.conj())
+    rho = sum(_kron_op({0: E}, n, D) @ rho @ _kron_op({0: E}, n, D).conj().T for E in fo.loss_kraus(0.6, D))
+    S1 = _kron_op({1: fo.squeezing(0.2, 0.5, D)}, n, D)
+    rho = S1 @ rho @ S1.conj().T
+    rho = sum(_kron_op({1: E}, n, D) @ rho @ _kron_op({1: E}, n, D).conj().T for E in fo.loss_kraus(0.8, D))
+    got = st.transpose(0, 2, 1, 3).reshape(D * D, D * D)
+    assert np.abs(got - rho).max() < 1e-13
+
+
+# ------------------------------------------------------------------ known answers
+# restated from the reference's tests/backend at tight tolerances
+@pytest.mark.parametrize("r,p", [(0.0, 0.0), (0.05, 1.0), (0.1, 2.5)])
+def test_coherent_state_fock_elements(r, p):
+    """test_displacement_operation.py:72-94."""
+    D = 12
+    st, _ = fo.run_ops([("D", (0,), (r, p))], 1, D)
+    a = r * np.exp(1j * p)
+    n = np.arange(D)
+    ref = np.exp(-0.5 * abs(a) ** 2) * a**n / np.sqrt(factorial(n))
+    assert np.abs(st - ref).max() < 1e-14
+
+
+@pytest.mark.parametrize("r,p", [(0.05, 0.0), (0.15, 2.0)])
+def test_squeezed_vacuum_loudon(r, p):
+    """test_squeeze_operation.py:104-133 (Loudon 5.5.6)."""
+    D = 10
+    st, _ = fo.run_ops([("S", (0,), (r, p))], 1, D)
+    k = np.arange(0, D, 2)
+    ref = np.sqrt(1 / np.cosh(r)) * np.sqrt(factorial(k)) / factorial(k / 2) * (-0.5 * np.exp(1j * p) * np.tanh(r)) ** (k / 2)
+    assert np.abs(st[::2] - ref).max() < 1e-14
+    assert np.all(st[1::2] == 0)
+
+
+def test_squeezed_fock_states():
+    """test_squeeze_operation.py:135-164 (arXiv quant-ph/0108024 Eq. 20)."""
+    from scipy.special import gammaln as lg
+
+    def matrix_elem(n, r, m):
+        if n % 2 != m % 2:
+            return 0.0
+        if r == 0.0:
+            return complex(n == m)
+        k = np.arange(m % 2, min([m, n]) + 1, 2)
+        return np.sum(
+            (-1) ** ((n - k) / 2)
+            * np.exp((lg(m + 1) + lg(n + 1)) / 2 - lg(k + 1) - lg((m - k) / 2 + 1) - lg((n - k) / 2 + 1))
+            * (np.sinh(r) / 2) ** ((n + m - 2 * k) / 2)
+            / (np.cosh(r) ** ((n + m + 1) / 2))
+        )
+
+    D, r = 8, 0.12
+    S = fo.squeezing(r, 0.0, D)
+    for m in range(D):
+        ref = np.array([matrix_elem(n, r, m) for n in range(D)])
+        assert np.abs(S[:, m] - ref).max() < 1e-13
+
+
+@pytest.mark.parametrize("modes", list(itertools.combinations(range(4), 2)))
+def test_beamsplitter_coherent_vacuum(modes):
+    """test_beamsplitter_operation.py:139-179: |a>|0> -> |t a>|r a> on a mode subset."""
+    D, t, r_phi, mag = 12, 0.6, 2.0, 0.1
+    alpha = mag * np.exp(1j * np.pi / 5)
+    r = np.exp(1j * r_phi) * np.sqrt(1 - t**2)
+    ops = [("D", (modes[0],), (mag, np.pi / 5)), ("BS", modes, (np.arccos(t), r_phi))]
+    st, pure = fo.run_ops(ops, 4, D)
+    red = fo.reduced_dm(st, 4, pure, list(modes))
+    n = np.arange(D)
+    A = np.exp(-0.5 * abs(t * alpha) ** 2) * (t * alpha) ** n / np.sqrt(factorial(n))
+    B = np.exp(-0.5 * abs(r * alpha) ** 2) * (r * alpha) ** n / np.sqrt(factorial(n))
+    ref = np.einsum("i,j,k,l->ijkl", A, A.conj(), B, B.conj())
+    assert np.abs(red - ref).max() < 1e-13
+
+
+def test_hong_ou_mandel():
+    """test_tf_integration.py:522-530: |1,1> through a 50:50 BS has p(1,1) = cos^2(2 theta)."""
+    D = 5
+    for theta in [np.pi / 4, 0.3]:
+        psi = np.zeros((D, D), dtype=complex)
+        psi[1, 1] = 1
+        st, _ = fo.run_ops([("BS", (0, 1), (theta, 0.4))], 2, D, state=psi)
+        assert abs(abs(st[1, 1]) ** 2 - np.cos(2 * theta) ** 2) < 1e-14
+        assert abs(abs(st[0, 2]) ** 2 - np.sin(2 * theta) ** 2 / 2) < 1e-14
+
+
+def test_kerr_on_uniform_state():
+    """test_nongaussian_gates.py:63-77."""
+    D, kappa = 7, 0.21
+    st, _ = fo.run_ops([("K", (0,), (kappa,))], 1, D, state=np.ones(D) / D)
+    assert np.abs(st - np.exp(1j * kappa * np.arange(D) ** 2) / D).max() < 1e-15
+
+
+@pytest.mark.parametrize("T", [0.0, 0.25, 0.5, 1.0])
+def test_loss_on_coherent_state(T):
+    """test_loss_channel.py:186-207 (rho -> |sqrt(T) a><sqrt(T) a|) and T=0 -> vacuum."""
+    D, mag, ph = 16, 0.2, 0.7
+    a0 = mag * np.exp(1j * ph)
+    n = np.arange(D)
+    coh = np.exp(-0.5 * abs(a0) ** 2) * a0**n / np.sqrt(factorial(n))
+    st, _ = fo.run_ops([("LOSS", (0,), (T,))], 1, D, state=coh)
+    a = np.sqrt(T) * a0
+    ref = np.exp(-0.5 * abs(a) ** 2) * a**n / np.sqrt(factorial(n))
+    assert np.abs(st - np.outer(ref, ref.conj())).max() < 1e-13
+
+
+def test_gate_then_inverse_returns_vacuum():
+    """test_ops_integration.py:53-73 (G then G.H restores vacuum) for the exactly invertible gates."""
+    D = 10
+    for name, p in [("R", (0.5,)), ("K", (0.3,)), ("BS", (0.4, 0.2))]:
+        n = 2 if name == "BS" else 1
+        modes = (0, 1) if name == "BS" else (0,)
+        start = [("D", (0,), (0.1, 0.3))]
+        p_inv = (-p[0],) + tuple(p[1:])
+        st_a, _ = fo.run_ops(start + [(name, modes, p), (name, modes, p_inv)], n, D)
+        st_b, _ = fo.run_ops(start, n, D)
+        assert np.abs(st_a - st_b).max() < 1e-14
+
+
+# ------------------------------------------------------------------ readout
+def test_readouts_pure_vs_mixed():
+    from strawberryfields_amd import circuits
+
+    c1 = circuits.config1()
+    psi, _ = fo.run_ops(c1["ops"], 3, 8)
+    rho = fo.mix(psi, 3)
+    assert abs(fo.trace(psi, 3, True) - fo.trace(rho, 3, False)) < 1e-14
+    assert np.abs(fo.all_fock_probs(psi, 3, True) - fo.all_fock_probs(rho, 3, False)).max() < 1e-15
+    for modes in ([0], [1], [2], [0, 2], [1, 2]):
+        a = fo.reduced_dm(psi, 3, True, modes)
+        b = fo.reduced_dm(rho, 3, False, modes)
+        assert np.abs(a - b).max() < 1e-15
+    m, v = fo.mean_photon(psi, 3, True, 1)
+    p = fo.all_fock_probs(psi, 3, True)
+    assert abs(m - (p.sum(axis=(0, 2)) * np.arange(8)).sum()) < 1e-14
+    e = fo.diagonal_expectation(psi, 3, True, [0, 2], np.arange(8))
+    w = np.arange(8)[:, None, None] * np.arange(8)[None, None, :]
+    assert abs(e - (p * w).sum()) < 1e-13
+    assert abs(fo.fock_prob(psi, 3, True, [1, 0, 2]) - p[1, 0, 2]) < 1e-16
+
+
+def test_coherent_mean_photon():
+    D, r = 20, 0.8
+    st, pure = fo.run_ops([("D", (0,), (r, 0.1))], 1, D)
+    m, v = fo.mean_photon(st, 1, pure, 0)
+    assert abs(m - r**2) < 1e-10 and abs(v - r**2) < 1e-9
