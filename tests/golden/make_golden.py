@@ -101,7 +101,7 @@ def bs_mp(theta, phi, D):
 CASES = {
     "D": [(0.3, 0.7, 6), (-0.45, 2.1, 12), (0.9, -1.3, 20), (0.05, 0.0, 15)],
     "S": [(0.3, 0.7, 6), (-0.4, 2.1, 12), (0.8, -1.3, 20), (0.1, 0.0, 15)],
-    "BS": [(0.6, 0.4, 5), (1.1, -2.3, 8), (-0.35, 1.7, 12)],
+    "BS": [(0.6, 0.4, 5), (1.1, -2.3, 8), (-0.35, 1.7, 12), (0.6, 0.4, 20)],
 }
 
 

