@@ -154,6 +154,19 @@ int sfamd_diagonal_expectation(const void* state, int n, int D, int mixed,
 int sfamd_all_fock_probs_vjp(const void* state, const void* g_probs, void* g_state,
                              int n, int D, int mixed, int dtype, void* stream);
 
+/* <a|b> over n_elems amplitudes with FP64 accumulation: out_dev[0] = Re,
+ * out_dev[1] = Im, out_dev[2] = 1 - |<a|b>|^2.  `partial` holds
+ * sfamd_overlap_partials(n_elems) doubles.  (BaseFockState.fidelity-style
+ * readout, states.py:665-730, for a full-register target ket.) */
+int64_t sfamd_overlap_partials(int64_t n_elems);
+int sfamd_overlap(const void* a, const void* b, int64_t n_elems, double* partial,
+                  double* out_dev, int dtype, void* stream);
+/* As sfamd_overlap with a = target, b = psi, and, if g_psi != NULL, the
+ * cotangent of L = 1 - |<target|psi>|^2: g_psi = -2 <target|psi> target. */
+int sfamd_fidelity_loss(const void* target, const void* psi, int64_t n_elems,
+                        double* partial, double* out_dev, void* g_psi,
+                        int dtype, void* stream);
+
 /* ---- recorded circuits (forward, adjoint, graphs) ----------------------- */
 typedef struct {
   int32_t kind;   /* SFAMD_OP_*                                              */
@@ -171,17 +184,26 @@ typedef struct {
   int64_t final_elems;    /* elements of the output state                   */
   int32_t final_mixed;    /* 1 if the output state is a DM                  */
   int32_t nops;
+  int32_t nsteps;         /* execution steps (fused stages or single ops)   */
+  int32_t pad;
 } sfamd_prog_info_t;
 
 /* Plans a circuit of `nops` operations on an n-mode state at cutoff D,
  * starting pure (mixed_in = 0) or mixed.  A loss on a pure state inserts the
  * pure -> mixed conversion as Circuit._apply_channel does
- * (fockbackend/circuit.py:79-81).  Host memory only. */
+ * (fockbackend/circuit.py:79-81).  With fuse_K >= 2, runs of gates on a pure
+ * state are grouped into steps that each apply many gates to LDS tiles of
+ * fuse_K local modes (gates on disjoint modes commute, so a gate may move to
+ * an earlier step when no earlier gate shares a mode); fuse_K = 0 gives one
+ * step per operation.  Host memory only. */
 int sfamd_prog_create(const sfamd_op_t* ops, int nops, int n, int D,
-                      int mixed_in, int dtype, sfamd_prog_t** out);
+                      int mixed_in, int dtype, int fuse_K, sfamd_prog_t** out);
 int sfamd_prog_info(const sfamd_prog_t* prog, sfamd_prog_info_t* info);
-/* state elements at the input of operation i (i = nops: the output) */
+/* state elements at the input of step i (i = nsteps: the output) */
 int64_t sfamd_prog_state_elems(const sfamd_prog_t* prog, int i);
+/* operation indices of step `step` in execution order; returns their count */
+int sfamd_prog_step_ops(const sfamd_prog_t* prog, int step, int32_t* ops_out,
+                        int cap);
 int sfamd_prog_destroy(sfamd_prog_t* prog);
 
 /* Gate table of all operations from the device parameter vector
@@ -189,22 +211,22 @@ int sfamd_prog_destroy(sfamd_prog_t* prog);
 int sfamd_prog_build_table(const sfamd_prog_t* prog, const double* params,
                            void* table, void* stream);
 
-/* Runs operations [begin, end) on `in`, writing the result to `out`.
+/* Runs steps [begin, end) on `in`, writing the result to `out`.
  * saved[i - begin], if saved != NULL and the entry is non-NULL, receives the
- * input state of operation i (operations write their output straight into the
- * next saved slot, so checkpoints cost no copies; saved[0] may equal `in`).
+ * input state of step i (steps write their output straight into the next
+ * saved slot, so checkpoints cost no copies; saved[0] may equal `in`).
  * `ws` is scratch of sfamd_prog_info.ws_elems elements.  `in` is not
  * modified; out must not alias in. */
 int sfamd_prog_forward(const sfamd_prog_t* prog, int begin, int end,
                        const void* table, const void* in, void* out,
                        void* const* saved, void* ws, void* stream);
 
-/* Adjoint sweep over operations [begin, end) in reverse.  saved[i - begin]
- * holds the input state of operation i; g_out is the cotangent of the state
- * after operation end-1 (PyTorch convention dL/dRe + i dL/dIm); g_in receives
- * the cotangent of the state before operation begin.  Parameter gradients of
- * these operations are written to grads[2*i + slot] (double, device).
- * `partial` holds sfamd_prog_info.partial_doubles doubles. */
+/* Adjoint sweep over steps [begin, end) in reverse.  saved[i - begin] holds
+ * the input state of step i; g_out is the cotangent of the state after step
+ * end-1 (PyTorch convention dL/dRe + i dL/dIm); g_in receives the cotangent
+ * of the state before step begin.  The gradients of the operations of these
+ * steps are written to grads[2*op + slot] (double, device); other entries are
+ * left untouched.  `partial` holds sfamd_prog_info.partial_doubles doubles. */
 int sfamd_prog_adjoint(const sfamd_prog_t* prog, int begin, int end,
                        const void* table, void* const* saved, const void* g_out,
                        void* g_in, double* grads, double* partial, void* ws,

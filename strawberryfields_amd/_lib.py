@@ -47,6 +47,8 @@ class ProgInfo(ctypes.Structure):
         ("final_elems", ctypes.c_int64),
         ("final_mixed", ctypes.c_int32),
         ("nops", ctypes.c_int32),
+        ("nsteps", ctypes.c_int32),
+        ("pad", ctypes.c_int32),
     ]
 
 
@@ -76,9 +78,13 @@ SIGNATURES = {
         [_P, _I, _I, _I, ctypes.POINTER(_I), _I, _P, _P, _P, _I, _P],
     ),
     "sfamd_all_fock_probs_vjp": (_I, [_P, _P, _P, _I, _I, _I, _I, _P]),
-    "sfamd_prog_create": (_I, [ctypes.POINTER(OpDesc), _I, _I, _I, _I, _I, ctypes.POINTER(_P)]),
+    "sfamd_overlap_partials": (_I64, [_I64]),
+    "sfamd_overlap": (_I, [_P, _P, _I64, _P, _P, _I, _P]),
+    "sfamd_fidelity_loss": (_I, [_P, _P, _I64, _P, _P, _P, _I, _P]),
+    "sfamd_prog_create": (_I, [ctypes.POINTER(OpDesc), _I, _I, _I, _I, _I, _I, ctypes.POINTER(_P)]),
     "sfamd_prog_info": (_I, [_P, ctypes.POINTER(ProgInfo)]),
     "sfamd_prog_state_elems": (_I64, [_P, _I]),
+    "sfamd_prog_step_ops": (_I, [_P, _I, ctypes.POINTER(ctypes.c_int32), _I]),
     "sfamd_prog_destroy": (_I, [_P]),
     "sfamd_prog_build_table": (_I, [_P, _P, _P, _P]),
     "sfamd_prog_forward": (_I, [_P, _I, _I, _P, _P, _P, ctypes.POINTER(_P), _P, _P]),

@@ -131,14 +131,17 @@ int64_t table_size(int kind, int D);
 
 // gate-table builder: ops[i] has kind, table offset (complex elements) and the
 // two parameters read from params[2*pidx + slot]
+// one operation of a table build: kind and table offset; the operation's
+// parameters are params[2*(first + i) + slot]
 struct BuildDesc {
   int32_t kind;
-  int32_t pidx;  // parameters params[2*pidx + slot]; < 0: use p0, p1 below
-  int64_t off;
-  double p0, p1;
+  int32_t off;  // complex elements
 };
-constexpr int kMaxBuildPerLaunch = 96;  // kernel-argument space (4 KB)
+constexpr int kMaxBuildPerLaunch = 480;  // kernel-argument space (4 KB)
 struct BuildBatch {
+  int32_t first;  // program index of d[0]; < 0: single gate with inline p0, p1
+  int32_t pad;
+  double p0, p1;
   BuildDesc d[kMaxBuildPerLaunch];
 };
 int launch_build(const BuildBatch& b, int count, int D, const double* params,
@@ -160,29 +163,34 @@ int replace_mode(const void* in, void* out, int64_t N, int D, int64_t sp,
                  int64_t sq, const void* sigma, int dtype, hipStream_t s);
 int fill_zero(void* p, int64_t bytes, hipStream_t s);
 
-// adjoint kernels; partial = base of this launch's [slot][nwg] doubles
+// adjoint kernels; partial = base of this launch's [slot][nwg] doubles; with
+// zp1 set the launch also writes zeros to the next pass block (partial + 2 nwg)
 int axis_dense_adj(const void* psi, const void* lam_out, void* lam_in, int64_t N,
                    int D, int64_t B, const void* M, const void* dM0,
-                   const void* dM1, int conj, double* partial, int nwg,
+                   const void* dM1, int conj, double* partial, int nwg, int zp1,
                    int dtype, hipStream_t s);
 int axis_diag_adj(const void* psi, const void* lam_out, void* lam_in, int64_t N,
                   int D, int64_t B, const void* d, int kerr, int conj,
-                  double* partial, int nwg, int dtype, hipStream_t s);
+                  double* partial, int nwg, int zp1, int dtype, hipStream_t s);
 int pair_diag_adj(const void* psi, const void* lam_out, void* lam_in, int64_t N,
                   int D, int64_t Bbra, int64_t Bket, const void* d, int kerr,
-                  double* partial, int nwg, int dtype, hipStream_t s);
+                  double* partial, int nwg, int zp1, int dtype, hipStream_t s);
 int pair_bs_adj(const void* psi, const void* lam_out, void* lam_in, int64_t N,
                 int D, int64_t sp, int64_t sq, const void* blocks,
-                const void* dblocks, int conj, double* partial, int nwg,
+                const void* dblocks, int conj, double* partial, int nwg, int zp1,
                 int dtype, hipStream_t s);
 int loss_adj(const void* psi, const void* lam_out, void* lam_in, int64_t N,
              int D, int64_t sp, int64_t sq, const void* coef, const void* dcoef,
-             double* partial, int nwg, int dtype, hipStream_t s);
+             double* partial, int nwg, int zp1, int dtype, hipStream_t s);
 int mix_adj(const void* ket, const void* g_dm, void* g_ket, int n, int D,
             int dtype, hipStream_t s);
 // grads[2*op + slot] = sum over passes p<2 and w<nwg of
-// partial[((op*2 + p)*2 + slot)*nwg + w], for op in [begin, end)
-int reduce_grads(const double* partial, double* grads, int begin, int end,
-                 int nwg, hipStream_t s);
+// partial[((op*2 + p)*2 + slot)*nwg + w], for the listed ops
+constexpr int kOpListMax = 960;
+struct OpList {
+  int32_t n;
+  int32_t ops[kOpListMax];
+};
+int reduce_grads(const double* partial, double* grads, const OpList& ol, int nwg, hipStream_t s);
 
 }  // namespace sfamd

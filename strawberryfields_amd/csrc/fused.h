@@ -1,0 +1,55 @@
+// Fused pure-state gate stages: descriptors shared by the planner
+// (program.hip) and the kernels (fused.hip).
+#pragma once
+
+#include "common.h"
+
+namespace sfamd {
+
+constexpr int kStageMaxGates = 40;   // gates per stage (kernel-argument space)
+constexpr int kStageMaxTile = 4096;  // amplitudes per LDS tile (D^K)
+constexpr int kStageMaxD = 20;       // cutoffs with compiled stage kernels: 2..20
+constexpr int kStageMaxModes = 24;   // modes outside the tile
+// dynamic LDS a stage may use: gfx950 has 160 KB of LDS per compute unit
+constexpr int64_t kStageLdsBudget = 152 * 1024;
+
+struct StageGate {
+  int8_t kind;    // SFAMD_OP_ROTATION .. SFAMD_OP_BEAMSPLITTER
+  int8_t a0;      // local axis of the gate's first mode (0 = slowest)
+  int8_t a1;      // local axis of the second mode (beamsplitter)
+  int8_t pad;
+  int32_t op;     // program operation index (gradient slot)
+  int64_t off;    // gate-table offset, complex elements
+  int32_t moff;   // LDS offset of the matrix, forward layout (matrix only)
+  int32_t amoff;  // LDS offset of matrix + derivatives, adjoint layout
+};
+
+struct StageArgs {
+  int32_t ngates;
+  int32_t nnl;        // modes outside the tile
+  int64_t ntiles;     // D^nnl
+  int32_t mat_fwd;    // LDS matrix region of the forward kernel (complex elements)
+  int32_t mat_adj;    // LDS matrix region of the adjoint kernel (complex elements)
+  int64_t loc_stride[4];               // global strides of the local modes
+  int64_t nl_stride[kStageMaxModes];   // global strides of the other modes (slowest first)
+  StageGate g[kStageMaxGates];
+};
+
+// threads per workgroup of the stage kernels: one tile element per thread up
+// to 1024 (more waves in flight per SIMD to hide LDS latency), at least 256
+__host__ __device__ constexpr int stage_threads(int DK) {
+  return DK >= 1024 ? 1024 : (DK <= 256 ? 256 : ((DK + 63) / 64) * 64);
+}
+
+bool stage_supported(int D, int K);
+// matrix + derivative table elements of one gate (the gate-table entry)
+int64_t stage_gate_elems(int kind, int D, bool derivs);
+// dynamic LDS bytes the stage kernels need
+int64_t stage_lds_fwd(int D, int K, int mat_fwd, int dtype);
+int64_t stage_lds_adj(int D, int K, int ngates, int mat_adj, int dtype);
+int stage_fwd(int D, int K, const void* in, void* out, const void* table, const StageArgs& a, int grid, int dtype,
+              hipStream_t s);
+int stage_adj(int D, int K, const void* psi, const void* lam_out, void* lam_in, const void* table,
+              const StageArgs& a, int grid, double* partial, int nwg, int dtype, hipStream_t s);
+
+}  // namespace sfamd

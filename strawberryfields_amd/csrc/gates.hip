@@ -317,10 +317,11 @@ __global__ void __launch_bounds__(256) k_build(sfamd::BuildBatch b, int D, const
   __shared__ cd sm_bs[2][kBSMax * kBSMax];
   __shared__ cd sm_col[3][3][SFAMD_MAX_CUTOFF];
   const sfamd::BuildDesc d = b.d[blockIdx.x];
-  double p0 = d.p0, p1 = d.p1;
-  if (d.pidx >= 0) {
-    p0 = params[2 * (int64_t)d.pidx];
-    p1 = params[2 * (int64_t)d.pidx + 1];
+  double p0 = b.p0, p1 = b.p1;
+  if (b.first >= 0) {
+    const int64_t pidx = (int64_t)b.first + blockIdx.x;
+    p0 = params[2 * pidx];
+    p1 = params[2 * pidx + 1];
   }
   cpx<T>* out = table + d.off;
   switch (d.kind) {

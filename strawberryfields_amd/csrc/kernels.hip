@@ -155,13 +155,17 @@ __global__ void __launch_bounds__(256) k_fill_zero(uint32_t* p, int64_t words) {
 // ------------------------------------------------------------------ adjoint
 // Each adjoint launch runs exactly nwg workgroups and writes its workgroup
 // partial sums of the two parameter slots to partial[slot * nwg + blockIdx.x].
-__device__ __forceinline__ void write_partials(double g0, double g1, double* partial, int nwg) {
+__device__ __forceinline__ void write_partials(double g0, double g1, double* partial, int nwg, int zp1) {
   __shared__ double sh[8];
   const double s0 = block_sum(g0, sh);
   const double s1 = block_sum(g1, sh);
   if (threadIdx.x == 0) {
     partial[blockIdx.x] = s0;
     partial[nwg + blockIdx.x] = s1;
+    if (zp1) {
+      partial[2 * nwg + blockIdx.x] = 0.0;
+      partial[3 * nwg + blockIdx.x] = 0.0;
+    }
   }
 }
 
@@ -169,7 +173,7 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 k_axis_dense_adj(const C<T>* __restrict__ psi, const C<T>* __restrict__ lo_, C<T>* __restrict__ li,
                  int64_t N, int D, int64_t B, const C<T>* __restrict__ M, const C<T>* __restrict__ dM0,
-                 const C<T>* __restrict__ dM1, int conj, double* partial, int nwg) {
+                 const C<T>* __restrict__ dM1, int conj, double* partial, int nwg, int zp1) {
   double g0 = 0.0, g1 = 0.0;
   GRID_STRIDE(e, N) {
     const int i = (int)((e / B) % D);
@@ -203,13 +207,13 @@ k_axis_dense_adj(const C<T>* __restrict__ psi, const C<T>* __restrict__ lo_, C<T
     g0 += (double)re_conj_mul(l, u0);
     g1 += (double)re_conj_mul(l, u1);
   }
-  write_partials(g0, g1, partial, nwg);
+  write_partials(g0, g1, partial, nwg, zp1);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_axis_diag_adj(const C<T>* __restrict__ psi, const C<T>* __restrict__ lo_, C<T>* __restrict__ li, int64_t N,
-                int D, int64_t B, const C<T>* __restrict__ d, int kerr, int conj, double* partial, int nwg) {
+                int D, int64_t B, const C<T>* __restrict__ d, int kerr, int conj, double* partial, int nwg, int zp1) {
   double g0 = 0.0;
   GRID_STRIDE(e, N) {
     const int i = (int)((e / B) % D);
@@ -223,14 +227,14 @@ k_axis_diag_adj(const C<T>* __restrict__ psi, const C<T>* __restrict__ lo_, C<T>
     const C<T> u = mk<T>(-w * fx.im, w * fx.re);
     g0 += (double)re_conj_mul(l, u);
   }
-  write_partials(g0, 0.0, partial, nwg);
+  write_partials(g0, 0.0, partial, nwg, zp1);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_pair_diag_adj(const C<T>* __restrict__ psi, const C<T>* __restrict__ lo_, C<T>* __restrict__ li, int64_t N,
                 int D, int64_t Bbra, int64_t Bket, const C<T>* __restrict__ d, int kerr, double* partial,
-                int nwg) {
+                int nwg, int zp1) {
   double g0 = 0.0;
   GRID_STRIDE(e, N) {
     const int i = (int)((e / Bbra) % D);
@@ -243,14 +247,14 @@ k_pair_diag_adj(const C<T>* __restrict__ psi, const C<T>* __restrict__ lo_, C<T>
     const C<T> u = mk<T>(-w * fx.im, w * fx.re);
     g0 += (double)re_conj_mul(l, u);
   }
-  write_partials(g0, 0.0, partial, nwg);
+  write_partials(g0, 0.0, partial, nwg, zp1);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_pair_bs_adj(const C<T>* __restrict__ psi, const C<T>* __restrict__ lo_, C<T>* __restrict__ li, int64_t N,
               int D, int64_t sp, int64_t sq, const C<T>* __restrict__ blk, const C<T>* __restrict__ dblk,
-              int conj, double* partial, int nwg) {
+              int conj, double* partial, int nwg, int zp1) {
   double g0 = 0.0, g1 = 0.0;
   GRID_STRIDE(e, N) {
     const int i = (int)((e / sp) % D);
@@ -291,14 +295,14 @@ k_pair_bs_adj(const C<T>* __restrict__ psi, const C<T>* __restrict__ lo_, C<T>* 
     g0 += (double)re_conj_mul(l, ut);
     g1 += (double)re_conj_mul(l, up);
   }
-  write_partials(g0, g1, partial, nwg);
+  write_partials(g0, g1, partial, nwg, zp1);
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_loss_adj(const C<T>* __restrict__ psi, const C<T>* __restrict__ lo_, C<T>* __restrict__ li, int64_t N, int D,
            int64_t sp, int64_t sq, const T* __restrict__ coef, const T* __restrict__ dcoef, double* partial,
-           int nwg) {
+           int nwg, int zp1) {
   double g0 = 0.0;
   GRID_STRIDE(e, N) {
     const int p = (int)((e / sp) % D);
@@ -324,7 +328,7 @@ k_loss_adj(const C<T>* __restrict__ psi, const C<T>* __restrict__ lo_, C<T>* __r
     }
     g0 += (double)re_conj_mul(lo_[e], u);
   }
-  write_partials(g0, 0.0, partial, nwg);
+  write_partials(g0, 0.0, partial, nwg, zp1);
 }
 
 // g_psi[I] = sum_J g[I, J] psi[J] + conj(g[J, I]) psi[J]  (rho = psi psi^dagger)
@@ -373,9 +377,9 @@ k_mix_adj(const C<T>* __restrict__ psi, const C<T>* __restrict__ g, C<T>* __rest
 }
 
 __global__ void __launch_bounds__(256)
-k_reduce_grads(const double* __restrict__ partial, double* __restrict__ grads, int begin, int nwg) {
+k_reduce_grads(const double* __restrict__ partial, double* __restrict__ grads, sfamd::OpList ol, int nwg) {
   __shared__ double sh[8];
-  const int op = begin + blockIdx.x / 2, slot = blockIdx.x % 2;
+  const int op = ol.ops[blockIdx.x / 2], slot = blockIdx.x % 2;
   double v = 0.0;
   for (int p = 0; p < 2; ++p) {
     const double* src = partial + (((int64_t)op * 2 + p) * 2 + slot) * nwg;
@@ -573,6 +577,53 @@ k_diag_expect(const C<T>* __restrict__ st, ModeList ml, int D, int mixed, int64_
   if (threadIdx.x == 0) partial[blockIdx.x] = s;
 }
 
+// <a|b> with FP64 accumulation: per-block partial (re, im) pairs
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_overlap_partial(const C<T>* __restrict__ a, const C<T>* __restrict__ b, int64_t N, double* partial) {
+  __shared__ double sh[8];
+  double vr = 0.0, vi = 0.0;
+  GRID_STRIDE(e, N) {
+    const C<T> x = a[e], y = b[e];
+    vr += (double)x.re * y.re + (double)x.im * y.im;
+    vi += (double)x.re * y.im - (double)x.im * y.re;
+  }
+  const double sr = block_sum(vr, sh);
+  const double si = block_sum(vi, sh);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = sr;
+    partial[2 * blockIdx.x + 1] = si;
+  }
+}
+
+// out[0..1] = <a|b>; out[2] = 1 - |<a|b>|^2
+__global__ void __launch_bounds__(256) k_overlap_final(const double* partial, int cnt, double* out) {
+  __shared__ double sh[8];
+  double vr = 0.0, vi = 0.0;
+  for (int w = threadIdx.x; w < cnt; w += blockDim.x) {
+    vr += partial[2 * w];
+    vi += partial[2 * w + 1];
+  }
+  const double sr = block_sum(vr, sh);
+  const double si = block_sum(vi, sh);
+  if (threadIdx.x == 0) {
+    out[0] = sr;
+    out[1] = si;
+    out[2] = 1.0 - sr * sr - si * si;
+  }
+}
+
+// cotangent of L = 1 - |<t|psi>|^2 with respect to psi: g = -2 <t|psi> t
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_fidelity_grad(const C<T>* __restrict__ t, const double* __restrict__ ov, C<T>* __restrict__ g, int64_t N) {
+  const T cr = (T)(-2.0 * ov[0]), ci = (T)(-2.0 * ov[1]);
+  GRID_STRIDE(e, N) {
+    const C<T> x = t[e];
+    g[e] = mk<T>(cr * x.re - ci * x.im, cr * x.im + ci * x.re);
+  }
+}
+
 // ------------------------------------------------------------------ helpers
 #define CP(T, p) ((const C<T>*)(p))
 #define MP(T, p) ((C<T>*)(p))
@@ -679,59 +730,59 @@ int fill_zero(void* p, int64_t bytes, hipStream_t s) {
 }
 
 int axis_dense_adj(const void* psi, const void* lam_out, void* lam_in, int64_t N, int D, int64_t B, const void* M,
-                   const void* dM0, const void* dM1, int conj, double* partial, int nwg, int dtype, hipStream_t s) {
+                   const void* dM0, const void* dM1, int conj, double* partial, int nwg, int zp1, int dtype, hipStream_t s) {
   if (dtype == SFAMD_C64)
     hipLaunchKernelGGL(k_axis_dense_adj<float>, dim3(nwg), dim3(kBlock), 0, s, CP(float, psi), CP(float, lam_out),
-                       MP(float, lam_in), N, D, B, CP(float, M), CP(float, dM0), CP(float, dM1), conj, partial, nwg);
+                       MP(float, lam_in), N, D, B, CP(float, M), CP(float, dM0), CP(float, dM1), conj, partial, nwg, zp1);
   else
     hipLaunchKernelGGL(k_axis_dense_adj<double>, dim3(nwg), dim3(kBlock), 0, s, CP(double, psi), CP(double, lam_out),
                        MP(double, lam_in), N, D, B, CP(double, M), CP(double, dM0), CP(double, dM1), conj, partial,
-                       nwg);
+                       nwg, zp1);
   return (int)hipGetLastError();
 }
 
 int axis_diag_adj(const void* psi, const void* lam_out, void* lam_in, int64_t N, int D, int64_t B, const void* d,
-                  int kerr, int conj, double* partial, int nwg, int dtype, hipStream_t s) {
+                  int kerr, int conj, double* partial, int nwg, int zp1, int dtype, hipStream_t s) {
   if (dtype == SFAMD_C64)
     hipLaunchKernelGGL(k_axis_diag_adj<float>, dim3(nwg), dim3(kBlock), 0, s, CP(float, psi), CP(float, lam_out),
-                       MP(float, lam_in), N, D, B, CP(float, d), kerr, conj, partial, nwg);
+                       MP(float, lam_in), N, D, B, CP(float, d), kerr, conj, partial, nwg, zp1);
   else
     hipLaunchKernelGGL(k_axis_diag_adj<double>, dim3(nwg), dim3(kBlock), 0, s, CP(double, psi), CP(double, lam_out),
-                       MP(double, lam_in), N, D, B, CP(double, d), kerr, conj, partial, nwg);
+                       MP(double, lam_in), N, D, B, CP(double, d), kerr, conj, partial, nwg, zp1);
   return (int)hipGetLastError();
 }
 
 int pair_diag_adj(const void* psi, const void* lam_out, void* lam_in, int64_t N, int D, int64_t Bbra, int64_t Bket,
-                  const void* d, int kerr, double* partial, int nwg, int dtype, hipStream_t s) {
+                  const void* d, int kerr, double* partial, int nwg, int zp1, int dtype, hipStream_t s) {
   if (dtype == SFAMD_C64)
     hipLaunchKernelGGL(k_pair_diag_adj<float>, dim3(nwg), dim3(kBlock), 0, s, CP(float, psi), CP(float, lam_out),
-                       MP(float, lam_in), N, D, Bbra, Bket, CP(float, d), kerr, partial, nwg);
+                       MP(float, lam_in), N, D, Bbra, Bket, CP(float, d), kerr, partial, nwg, zp1);
   else
     hipLaunchKernelGGL(k_pair_diag_adj<double>, dim3(nwg), dim3(kBlock), 0, s, CP(double, psi), CP(double, lam_out),
-                       MP(double, lam_in), N, D, Bbra, Bket, CP(double, d), kerr, partial, nwg);
+                       MP(double, lam_in), N, D, Bbra, Bket, CP(double, d), kerr, partial, nwg, zp1);
   return (int)hipGetLastError();
 }
 
 int pair_bs_adj(const void* psi, const void* lam_out, void* lam_in, int64_t N, int D, int64_t sp, int64_t sq,
-                const void* blocks, const void* dblocks, int conj, double* partial, int nwg, int dtype,
+                const void* blocks, const void* dblocks, int conj, double* partial, int nwg, int zp1, int dtype,
                 hipStream_t s) {
   if (dtype == SFAMD_C64)
     hipLaunchKernelGGL(k_pair_bs_adj<float>, dim3(nwg), dim3(kBlock), 0, s, CP(float, psi), CP(float, lam_out),
-                       MP(float, lam_in), N, D, sp, sq, CP(float, blocks), CP(float, dblocks), conj, partial, nwg);
+                       MP(float, lam_in), N, D, sp, sq, CP(float, blocks), CP(float, dblocks), conj, partial, nwg, zp1);
   else
     hipLaunchKernelGGL(k_pair_bs_adj<double>, dim3(nwg), dim3(kBlock), 0, s, CP(double, psi), CP(double, lam_out),
-                       MP(double, lam_in), N, D, sp, sq, CP(double, blocks), CP(double, dblocks), conj, partial, nwg);
+                       MP(double, lam_in), N, D, sp, sq, CP(double, blocks), CP(double, dblocks), conj, partial, nwg, zp1);
   return (int)hipGetLastError();
 }
 
 int loss_adj(const void* psi, const void* lam_out, void* lam_in, int64_t N, int D, int64_t sp, int64_t sq,
-             const void* coef, const void* dcoef, double* partial, int nwg, int dtype, hipStream_t s) {
+             const void* coef, const void* dcoef, double* partial, int nwg, int zp1, int dtype, hipStream_t s) {
   if (dtype == SFAMD_C64)
     hipLaunchKernelGGL(k_loss_adj<float>, dim3(nwg), dim3(kBlock), 0, s, CP(float, psi), CP(float, lam_out),
-                       MP(float, lam_in), N, D, sp, sq, (const float*)coef, (const float*)dcoef, partial, nwg);
+                       MP(float, lam_in), N, D, sp, sq, (const float*)coef, (const float*)dcoef, partial, nwg, zp1);
   else
     hipLaunchKernelGGL(k_loss_adj<double>, dim3(nwg), dim3(kBlock), 0, s, CP(double, psi), CP(double, lam_out),
-                       MP(double, lam_in), N, D, sp, sq, (const double*)coef, (const double*)dcoef, partial, nwg);
+                       MP(double, lam_in), N, D, sp, sq, (const double*)coef, (const double*)dcoef, partial, nwg, zp1);
   return (int)hipGetLastError();
 }
 
@@ -747,9 +798,9 @@ int mix_adj(const void* ket, const void* g_dm, void* g_ket, int n, int D, int dt
   return (int)hipGetLastError();
 }
 
-int reduce_grads(const double* partial, double* grads, int begin, int end, int nwg, hipStream_t s) {
-  if (end <= begin) return 0;
-  hipLaunchKernelGGL(k_reduce_grads, dim3(2 * (end - begin)), dim3(kBlock), 0, s, partial, grads, begin, nwg);
+int reduce_grads(const double* partial, double* grads, const OpList& ol, int nwg, hipStream_t s) {
+  if (ol.n <= 0) return 0;
+  hipLaunchKernelGGL(k_reduce_grads, dim3(2 * ol.n), dim3(kBlock), 0, s, partial, grads, ol, nwg);
   return (int)hipGetLastError();
 }
 
@@ -882,5 +933,37 @@ extern "C" int sfamd_diagonal_expectation(const void* state, int n, int D, int m
                        values_dev, partial);
   HIP_RET(hipGetLastError());
   hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, s, partial, g, out_dev);
+  return (int)hipGetLastError();
+}
+
+extern "C" int64_t sfamd_overlap_partials(int64_t n_elems) { return 2 * reduce_grid(n_elems); }
+
+extern "C" int sfamd_overlap(const void* a, const void* b, int64_t n_elems, double* partial, double* out_dev,
+                             int dtype, void* stream) {
+  CHECK_ARG(a && b && partial && out_dev && n_elems >= 0);
+  hipStream_t s = (hipStream_t)stream;
+  const int g = (int)reduce_grid(n_elems);
+  if (dtype == SFAMD_C64)
+    hipLaunchKernelGGL(k_overlap_partial<float>, dim3(g), dim3(kBlock), 0, s, CP(float, a), CP(float, b), n_elems,
+                       partial);
+  else
+    hipLaunchKernelGGL(k_overlap_partial<double>, dim3(g), dim3(kBlock), 0, s, CP(double, a), CP(double, b), n_elems,
+                       partial);
+  HIP_RET(hipGetLastError());
+  hipLaunchKernelGGL(k_overlap_final, dim3(1), dim3(kBlock), 0, s, partial, g, out_dev);
+  return (int)hipGetLastError();
+}
+
+extern "C" int sfamd_fidelity_loss(const void* target, const void* psi, int64_t n_elems, double* partial,
+                                   double* out_dev, void* g_psi, int dtype, void* stream) {
+  RET_IF(sfamd_overlap(target, psi, n_elems, partial, out_dev, dtype, stream));
+  if (!g_psi) return 0;
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == SFAMD_C64)
+    hipLaunchKernelGGL(k_fidelity_grad<float>, dim3(grid_for(n_elems)), dim3(kBlock), 0, s, CP(float, target),
+                       out_dev, MP(float, g_psi), n_elems);
+  else
+    hipLaunchKernelGGL(k_fidelity_grad<double>, dim3(grid_for(n_elems)), dim3(kBlock), 0, s, CP(double, target),
+                       out_dev, MP(double, g_psi), n_elems);
   return (int)hipGetLastError();
 }

@@ -6,10 +6,13 @@
 // state in caller-provided buffers, writes the inputs of requested operations
 // as checkpoints and runs the reverse sweep.  The whole sequence can be
 // captured into a HIP graph by the caller (sfamd_graph_*).
+#include <algorithm>
+#include <cstring>
 #include <new>
 #include <vector>
 
 #include "common.h"
+#include "fused.h"
 
 using namespace sfamd;
 
@@ -102,8 +105,8 @@ int adjoint_op(const OpPlan& op, int n, int D, const void* table, const void* ps
     case SFAMD_OP_KERR:
       if (op.mixed_in)
         return pair_diag_adj(psi, lo, li, NM, D, bra_stride_dm(n, D, op.m0), ket_stride_dm(n, D, op.m0), g, kerr,
-                             pass0, nwg, dtype, s);
-      return axis_diag_adj(psi, lo, li, NK, D, ket_stride(n, D, op.m0), g, kerr, 0, pass0, nwg, dtype, s);
+                             pass0, nwg, 1, dtype, s);
+      return axis_diag_adj(psi, lo, li, NK, D, ket_stride(n, D, op.m0), g, kerr, 0, pass0, nwg, 1, dtype, s);
     case SFAMD_OP_DISPLACEMENT:
     case SFAMD_OP_SQUEEZE: {
       const int64_t DD = (int64_t)D * D;
@@ -113,10 +116,10 @@ int adjoint_op(const OpPlan& op, int n, int D, const void* table, const void* ps
         const int64_t sb = bra_stride_dm(n, D, op.m0), sk = ket_stride_dm(n, D, op.m0);
         // rho1 = U rho (bra side) recomputed; then the ket-side pass, then the bra-side pass
         RET_IF(axis_dense(psi, t1, NM, D, sb, g, 0, dtype, s));
-        RET_IF(axis_dense_adj(t1, lo, t2, NM, D, sk, g, dU0, dU1, 1, pass1, nwg, dtype, s));
-        return axis_dense_adj(psi, t2, li, NM, D, sb, g, dU0, dU1, 0, pass0, nwg, dtype, s);
+        RET_IF(axis_dense_adj(t1, lo, t2, NM, D, sk, g, dU0, dU1, 1, pass1, nwg, 0, dtype, s));
+        return axis_dense_adj(psi, t2, li, NM, D, sb, g, dU0, dU1, 0, pass0, nwg, 0, dtype, s);
       }
-      return axis_dense_adj(psi, lo, li, NK, D, ket_stride(n, D, op.m0), g, dU0, dU1, 0, pass0, nwg, dtype, s);
+      return axis_dense_adj(psi, lo, li, NK, D, ket_stride(n, D, op.m0), g, dU0, dU1, 0, pass0, nwg, 1, dtype, s);
     }
     case SFAMD_OP_BEAMSPLITTER: {
       const void* dB = cptr(g, bs_nb(D), dtype);
@@ -124,11 +127,11 @@ int adjoint_op(const OpPlan& op, int n, int D, const void* table, const void* ps
         const int64_t bp = bra_stride_dm(n, D, op.m0), bq = bra_stride_dm(n, D, op.m1);
         const int64_t kp = ket_stride_dm(n, D, op.m0), kq = ket_stride_dm(n, D, op.m1);
         RET_IF(pair_bs(psi, t1, NM, D, bp, bq, g, 0, dtype, s));
-        RET_IF(pair_bs_adj(t1, lo, t2, NM, D, kp, kq, g, dB, 1, pass1, nwg, dtype, s));
-        return pair_bs_adj(psi, t2, li, NM, D, bp, bq, g, dB, 0, pass0, nwg, dtype, s);
+        RET_IF(pair_bs_adj(t1, lo, t2, NM, D, kp, kq, g, dB, 1, pass1, nwg, 0, dtype, s));
+        return pair_bs_adj(psi, t2, li, NM, D, bp, bq, g, dB, 0, pass0, nwg, 0, dtype, s);
       }
       return pair_bs_adj(psi, lo, li, NK, D, ket_stride(n, D, op.m0), ket_stride(n, D, op.m1), g, dB, 0, pass0, nwg,
-                         dtype, s);
+                         1, dtype, s);
     }
     case SFAMD_OP_LOSS: {
       const int64_t D3 = (int64_t)D * D * D;
@@ -137,12 +140,13 @@ int adjoint_op(const OpPlan& op, int n, int D, const void* table, const void* ps
       const int64_t sb = bra_stride_dm(n, D, op.m0), sk = ket_stride_dm(n, D, op.m0);
       if (op.mix_first) {
         RET_IF(mix_fwd(psi, t1, n, D, dtype, s));
-        RET_IF(loss_adj(t1, lo, t2, NM, D, sb, sk, c, dc, pass0, nwg, dtype, s));
+        RET_IF(loss_adj(t1, lo, t2, NM, D, sb, sk, c, dc, pass0, nwg, 1, dtype, s));
         return mix_adj(psi, t2, li, n, D, dtype, s);
       }
-      return loss_adj(psi, lo, li, NM, D, sb, sk, c, dc, pass0, nwg, dtype, s);
+      return loss_adj(psi, lo, li, NM, D, sb, sk, c, dc, pass0, nwg, 1, dtype, s);
     }
     case SFAMD_OP_MIX:
+      RET_IF(fill_zero(pb, 4 * (int64_t)nwg * sizeof(double), s));
       return mix_adj(psi, lo, li, n, D, dtype, s);
     default:
       return SFAMD_E_KIND;
@@ -151,13 +155,208 @@ int adjoint_op(const OpPlan& op, int n, int D, const void* table, const void* ps
 
 }  // namespace
 
+// One step of execution: a single operation (element-wise kernels) or a
+// fused stage of gates on K local modes of a pure state (fused.hip).
+struct Step {
+  int fused;
+  int op;  // single operation index (fused == 0)
+  int K;
+  std::vector<int> ops;  // operation indices in execution order
+  StageArgs args;
+};
+
 struct sfamd_prog {
-  int n, D, dtype, mixed_in, nops, nwg;
+  int n, D, dtype, mixed_in, nops, nwg, fuse_K;
   std::vector<OpPlan> ops;
-  std::vector<int64_t> state_elems;  // elements at the input of op i (i = nops: output)
+  std::vector<Step> steps;
+  std::vector<int64_t> state_elems;  // elements at the input of step i (i = nsteps: output)
   int64_t table_elems, max_state;
   int final_mixed;
 };
+
+namespace {
+
+bool fusable(const OpPlan& op) {
+  return !op.mixed_in && (op.kind == SFAMD_OP_ROTATION || op.kind == SFAMD_OP_KERR ||
+                          op.kind == SFAMD_OP_DISPLACEMENT || op.kind == SFAMD_OP_SQUEEZE ||
+                          op.kind == SFAMD_OP_BEAMSPLITTER);
+}
+
+int op_nmodes(const OpPlan& op) { return op.kind == SFAMD_OP_BEAMSPLITTER ? 2 : 1; }
+
+bool op_in(const OpPlan& op, unsigned mask) {
+  if (!(mask >> op.m0 & 1u)) return false;
+  return op.kind != SFAMD_OP_BEAMSPLITTER || (mask >> op.m1 & 1u);
+}
+
+unsigned op_mask(const OpPlan& op) {
+  unsigned m = 1u << op.m0;
+  if (op.kind == SFAMD_OP_BEAMSPLITTER) m |= 1u << op.m1;
+  return m;
+}
+
+// ops of `rem` (program order) absorbable into a stage on `mask`: an op joins
+// if its modes lie in the mask, no earlier op left outside the stage touches
+// one of them (gates on disjoint modes commute) and the adjoint kernel's LDS
+// (one tile per gate plus three, and every gate's matrix + derivatives) fits
+void absorb(const std::vector<OpPlan>& ops, const std::vector<int>& rem, unsigned mask, int D, int64_t tile_bytes,
+            int64_t esz, std::vector<int>* got) {
+  got->clear();
+  unsigned blocked = 0;
+  int64_t used = 3 * tile_bytes;
+  for (int i : rem) {
+    const OpPlan& op = ops[i];
+    const unsigned om = op_mask(op);
+    const int64_t cost = tile_bytes + stage_gate_elems(op.kind, D, true) * esz +
+                         2 * (stage_threads((int)(tile_bytes / esz)) / 64) * 8;
+    if ((int)got->size() < kStageMaxGates && op_in(op, mask) && !(om & blocked) && used + cost <= kStageLdsBudget) {
+      got->push_back(i);
+      used += cost;
+    } else {
+      blocked |= om;
+    }
+  }
+}
+
+// greedy partition of a run of fusable ops into stages of Kc local modes
+void plan_run(sfamd_prog* p, const std::vector<int>& run, int Kc) {
+  const int n = p->n, D = p->D;
+  const int64_t esz = elem_size(p->dtype);
+  const int64_t tile_bytes = ipow(D, Kc) * esz;
+  std::vector<int> rem = run;
+  std::vector<unsigned> cands;
+  // all Kc-subsets when few, else subsets grown from the next ops
+  std::vector<int> idx(Kc);
+  int64_t ncomb = 1;
+  for (int t = 0; t < Kc; ++t) ncomb = ncomb * (n - t) / (t + 1);
+  if (ncomb <= 512) {
+    for (int t = 0; t < Kc; ++t) idx[t] = t;
+    while (true) {
+      unsigned m = 0;
+      for (int t = 0; t < Kc; ++t) m |= 1u << idx[t];
+      cands.push_back(m);
+      int t = Kc - 1;
+      while (t >= 0 && idx[t] == n - Kc + t) --t;
+      if (t < 0) break;
+      ++idx[t];
+      for (int u = t + 1; u < Kc; ++u) idx[u] = idx[u - 1] + 1;
+    }
+  }
+  std::vector<int> got, best;
+  while (!rem.empty()) {
+    std::vector<unsigned> local = cands;
+    if (local.empty()) {
+      unsigned m = 0;
+      for (int i : rem) {
+        const unsigned om = op_mask(p->ops[i]);
+        if (__builtin_popcount(m | om) <= Kc) m |= om;
+        if (__builtin_popcount(m) == Kc) break;
+      }
+      for (int md = n - 1; __builtin_popcount(m) < Kc && md >= 0; --md) m |= 1u << md;
+      local.push_back(m);
+    }
+    unsigned bm = 0;
+    long bkey = -1;
+    for (unsigned m : local) {
+      absorb(p->ops, rem, m, D, tile_bytes, esz, &got);
+      // more gates first; then tiles that include the last modes (contiguous runs)
+      const long key = (long)got.size() * (1L << 26) + (long)m;
+      if (key > bkey) {
+        bkey = key;
+        bm = m;
+        best = got;
+      }
+    }
+    Step st;
+    st.fused = 1;
+    st.op = best.front();
+    st.K = Kc;
+    st.ops = best;
+    StageArgs& a = st.args;
+    memset(&a, 0, sizeof(a));
+    a.ngates = (int)best.size();
+    int lmodes[4], nl = 0, nlc = 0;
+    for (int md = 0; md < n; ++md) {
+      if (bm >> md & 1u)
+        lmodes[nl++] = md;
+      else
+        a.nl_stride[nlc++] = ipow(D, n - 1 - md);
+    }
+    a.nnl = nlc;
+    a.ntiles = ipow(D, nlc);
+    for (int t = 0; t < Kc; ++t) a.loc_stride[t] = ipow(D, n - 1 - lmodes[t]);
+    int64_t mf = 0, ma = 0;
+    for (int gi = 0; gi < a.ngates; ++gi) {
+      const OpPlan& op = p->ops[best[gi]];
+      StageGate& g = a.g[gi];
+      g.kind = (int8_t)op.kind;
+      g.a0 = g.a1 = 0;
+      for (int t = 0; t < Kc; ++t) {
+        if (lmodes[t] == op.m0) g.a0 = (int8_t)t;
+        if (op.kind == SFAMD_OP_BEAMSPLITTER && lmodes[t] == op.m1) g.a1 = (int8_t)t;
+      }
+      g.op = best[gi];
+      g.off = op.off;
+      g.moff = (int32_t)mf;
+      g.amoff = (int32_t)ma;
+      // keep each matrix 16-byte aligned in LDS
+      mf += (stage_gate_elems(op.kind, D, false) + 1) & ~(int64_t)1;
+      ma += (stage_gate_elems(op.kind, D, true) + 1) & ~(int64_t)1;
+    }
+    a.mat_fwd = (int32_t)mf;
+    a.mat_adj = (int32_t)ma;
+    p->steps.push_back(st);
+    std::vector<int> nrem;
+    size_t bi = 0;
+    for (int i : rem) {
+      if (bi < best.size() && best[bi] == i)
+        ++bi;
+      else
+        nrem.push_back(i);
+    }
+    rem.swap(nrem);
+  }
+}
+
+void plan_steps(sfamd_prog* p) {
+  const int n = p->n, D = p->D;
+  const int Kc = p->fuse_K < n ? p->fuse_K : n;
+  const int64_t esz = elem_size(p->dtype);
+  // fusion needs compiled stage kernels and room for at least one gate of any kind
+  int gmax = 0;
+  if (Kc >= 2 && n <= kStageMaxModes && stage_supported(D, Kc)) {
+    const int64_t one = 4 * ipow(D, Kc) * esz + stage_gate_elems(SFAMD_OP_BEAMSPLITTER, D, true) * esz + 256;
+    gmax = one <= kStageLdsBudget ? 1 : 0;
+  }
+  std::vector<int> run;
+  auto flush = [&]() {
+    if (!run.empty()) plan_run(p, run, Kc);
+    run.clear();
+  };
+  for (int i = 0; i < p->nops; ++i) {
+    if (gmax >= 1 && fusable(p->ops[i])) {
+      run.push_back(i);
+      continue;
+    }
+    flush();
+    Step st;
+    st.fused = 0;
+    st.op = i;
+    st.K = 0;
+    st.ops.push_back(i);
+    memset(&st.args, 0, sizeof(st.args));
+    p->steps.push_back(st);
+  }
+  flush();
+  // state sizes at step inputs
+  p->state_elems.clear();
+  for (const Step& st : p->steps) {
+    const OpPlan& op = p->ops[st.ops.front()];
+    p->state_elems.push_back(ipow(D, op.mixed_in ? 2 * n : n));
+  }
+}
+
+}  // namespace
 
 // ====================================================================== C ABI
 extern "C" int sfamd_version(void) { return SFAMD_VERSION; }
@@ -194,11 +393,11 @@ extern "C" int sfamd_build_gate(int kind, double p0, double p1, int D, int dtype
   RET_IF(validate_op(kind, 0, 1, 2, D));
   if (kind == SFAMD_OP_MIX) return SFAMD_E_KIND;
   BuildBatch b;
+  b.first = -1;
+  b.p0 = p0;
+  b.p1 = p1;
   b.d[0].kind = kind;
-  b.d[0].pidx = -1;
   b.d[0].off = 0;
-  b.d[0].p0 = p0;
-  b.d[0].p1 = p1;
   return launch_build(b, 1, D, nullptr, out, dtype, (hipStream_t)stream);
 }
 
@@ -247,7 +446,7 @@ extern "C" int sfamd_replace_mode(const void* in, void* out, int n, int D, int m
 }
 
 // ---------------------------------------------------------------- programs
-extern "C" int sfamd_prog_create(const sfamd_op_t* ops, int nops, int n, int D, int mixed_in, int dtype,
+extern "C" int sfamd_prog_create(const sfamd_op_t* ops, int nops, int n, int D, int mixed_in, int dtype, int fuse_K,
                                  sfamd_prog_t** out) {
   CHECK_ARG(out && nops >= 0 && n >= 1 && n <= 32 && (ops || nops == 0));
   CHECK_ARG(dtype == SFAMD_C64 || dtype == SFAMD_C128);
@@ -259,9 +458,9 @@ extern "C" int sfamd_prog_create(const sfamd_op_t* ops, int nops, int n, int D, 
   p->dtype = dtype;
   p->mixed_in = mixed_in ? 1 : 0;
   p->nops = nops;
+  p->fuse_K = fuse_K;
   int mixed = p->mixed_in;
   int64_t off = 0, max_state = ipow(D, mixed ? 2 * n : n);
-  p->state_elems.push_back(max_state);
   for (int i = 0; i < nops; ++i) {
     const int st = validate_op(ops[i].kind, ops[i].mode0, ops[i].mode1, n, D);
     if (st != 0) {
@@ -278,7 +477,6 @@ extern "C" int sfamd_prog_create(const sfamd_op_t* ops, int nops, int n, int D, 
     off += table_size(op.kind, D);
     p->ops.push_back(op);
     const int64_t se = ipow(D, mixed ? 2 * n : n);
-    p->state_elems.push_back(se);
     if (se > max_state) max_state = se;
     // a two-pass op on a DM and a pure->mixed loss need DM-sized scratch
     if (op.mix_first) max_state = ipow(D, 2 * n) > max_state ? ipow(D, 2 * n) : max_state;
@@ -286,6 +484,8 @@ extern "C" int sfamd_prog_create(const sfamd_op_t* ops, int nops, int n, int D, 
   p->table_elems = off;
   p->max_state = max_state;
   p->final_mixed = mixed;
+  plan_steps(p);
+  p->state_elems.push_back(ipow(D, mixed ? 2 * n : n));
   int64_t nwg = (max_state + kBlock - 1) / kBlock;
   if (nwg > 4096) nwg = 4096;
   if (nwg < 1) nwg = 1;
@@ -302,12 +502,20 @@ extern "C" int sfamd_prog_info(const sfamd_prog_t* p, sfamd_prog_info_t* info) {
   info->final_elems = p->state_elems.back();
   info->final_mixed = p->final_mixed;
   info->nops = p->nops;
+  info->nsteps = (int32_t)p->steps.size();
   return 0;
 }
 
 extern "C" int64_t sfamd_prog_state_elems(const sfamd_prog_t* p, int i) {
-  if (!p || i < 0 || i > p->nops) return -1;
+  if (!p || i < 0 || i > (int)p->steps.size()) return -1;
   return p->state_elems[i];
+}
+
+extern "C" int sfamd_prog_step_ops(const sfamd_prog_t* p, int step, int32_t* ops_out, int cap) {
+  if (!p || step < 0 || step >= (int)p->steps.size()) return SFAMD_E_ARG;
+  const Step& st = p->steps[step];
+  for (int i = 0; i < (int)st.ops.size() && i < cap; ++i) ops_out[i] = st.ops[i];
+  return (int)st.ops.size();
 }
 
 extern "C" int sfamd_prog_destroy(sfamd_prog_t* p) {
@@ -317,27 +525,25 @@ extern "C" int sfamd_prog_destroy(sfamd_prog_t* p) {
 
 extern "C" int sfamd_prog_build_table(const sfamd_prog_t* p, const double* params, void* table, void* stream) {
   CHECK_ARG(p && (p->nops == 0 || (params && table)));
+  CHECK_ARG(p->table_elems < ((int64_t)1 << 31));
   BuildBatch b;
-  int cnt = 0;
-  for (int i = 0; i < p->nops; ++i) {
-    const OpPlan& op = p->ops[i];
-    if (op.kind == SFAMD_OP_MIX) continue;
-    b.d[cnt].kind = op.kind;
-    b.d[cnt].pidx = i;
-    b.d[cnt].off = op.off;
-    b.d[cnt].p0 = 0.0;
-    b.d[cnt].p1 = 0.0;
-    if (++cnt == kMaxBuildPerLaunch) {
-      RET_IF(launch_build(b, cnt, p->D, params, table, p->dtype, (hipStream_t)stream));
-      cnt = 0;
+  b.p0 = b.p1 = 0.0;
+  for (int first = 0; first < p->nops; first += kMaxBuildPerLaunch) {
+    const int cnt = p->nops - first < kMaxBuildPerLaunch ? p->nops - first : kMaxBuildPerLaunch;
+    b.first = first;
+    for (int i = 0; i < cnt; ++i) {
+      b.d[i].kind = p->ops[first + i].kind;  // MIX entries build nothing
+      b.d[i].off = (int32_t)p->ops[first + i].off;
     }
+    RET_IF(launch_build(b, cnt, p->D, params, table, p->dtype, (hipStream_t)stream));
   }
-  return launch_build(b, cnt, p->D, params, table, p->dtype, (hipStream_t)stream);
+  return 0;
 }
 
 extern "C" int sfamd_prog_forward(const sfamd_prog_t* p, int begin, int end, const void* table, const void* in,
                                   void* out, void* const* saved, void* ws, void* stream) {
-  CHECK_ARG(p && in && out && in != out && 0 <= begin && begin <= end && end <= p->nops);
+  const int nsteps = p ? (int)p->steps.size() : 0;
+  CHECK_ARG(p && in && out && in != out && 0 <= begin && begin <= end && end <= nsteps);
   CHECK_ARG(ws || begin == end);
   hipStream_t s = (hipStream_t)stream;
   const int64_t esz = elem_size(p->dtype);
@@ -360,7 +566,13 @@ extern "C" int sfamd_prog_forward(const sfamd_prog_t* p, int begin, int end, con
     else
       dst = (cur == A) ? B : A;
     if (dst == cur) return SFAMD_E_ARG;
-    RET_IF(apply_op(p->ops[i], p->n, p->D, table, cur, dst, Ct, p->dtype, s));
+    const Step& st = p->steps[i];
+    if (st.fused) {
+      int grid = (int)(st.args.ntiles < p->nwg ? st.args.ntiles : p->nwg);
+      RET_IF(stage_fwd(p->D, st.K, cur, dst, table, st.args, grid, p->dtype, s));
+    } else {
+      RET_IF(apply_op(p->ops[st.op], p->n, p->D, table, cur, dst, Ct, p->dtype, s));
+    }
     cur = dst;
   }
   return 0;
@@ -369,8 +581,9 @@ extern "C" int sfamd_prog_forward(const sfamd_prog_t* p, int begin, int end, con
 extern "C" int sfamd_prog_adjoint(const sfamd_prog_t* p, int begin, int end, const void* table, void* const* saved,
                                   const void* g_out, void* g_in, double* grads, double* partial, void* ws,
                                   void* stream) {
+  const int nsteps = p ? (int)p->steps.size() : 0;
   CHECK_ARG(p && saved && g_out && g_in && grads && partial && ws && g_in != g_out);
-  CHECK_ARG(0 <= begin && begin < end && end <= p->nops);
+  CHECK_ARG(0 <= begin && begin < end && end <= nsteps);
   hipStream_t s = (hipStream_t)stream;
   const int64_t esz = elem_size(p->dtype);
   char* LA = (char*)ws;
@@ -378,16 +591,34 @@ extern "C" int sfamd_prog_adjoint(const sfamd_prog_t* p, int begin, int end, con
   char* T1 = LB + p->max_state * esz;
   char* T2 = T1 + p->max_state * esz;
   const int nwg = p->nwg;
-  RET_IF(fill_zero(partial + (int64_t)begin * 4 * nwg, (int64_t)(end - begin) * 4 * nwg * sizeof(double), s));
+  // every adjoint launch writes its operation's whole [pass][slot][nwg]
+  // partial block, so nothing needs clearing between calls
   const void* lam = g_out;
   for (int i = end - 1; i >= begin; --i) {
     if (!saved[i - begin]) return SFAMD_E_ARG;
     void* dst = (i == begin) ? g_in : ((lam == LA) ? LB : LA);
-    RET_IF(adjoint_op(p->ops[i], p->n, p->D, table, saved[i - begin], lam, dst, T1, T2,
-                      partial + (int64_t)i * 4 * nwg, nwg, p->dtype, s));
+    const Step& st = p->steps[i];
+    if (st.fused) {
+      RET_IF(stage_adj(p->D, st.K, saved[i - begin], lam, dst, table, st.args, nwg, partial, nwg, p->dtype, s));
+    } else {
+      RET_IF(adjoint_op(p->ops[st.op], p->n, p->D, table, saved[i - begin], lam, dst, T1, T2,
+                        partial + (int64_t)st.op * 4 * nwg, nwg, p->dtype, s));
+    }
     lam = dst;
   }
-  return reduce_grads(partial, grads, begin, end, nwg, s);
+  // gradients of the operations of these steps, one launch per OpList chunk
+  OpList ol;
+  ol.n = 0;
+  for (int i = begin; i < end; ++i) {
+    for (int op : p->steps[i].ops) {
+      ol.ops[ol.n++] = op;
+      if (ol.n == kOpListMax) {
+        RET_IF(reduce_grads(partial, grads, ol, nwg, s));
+        ol.n = 0;
+      }
+    }
+  }
+  return reduce_grads(partial, grads, ol, nwg, s);
 }
 
 // ---------------------------------------------------------------- graphs

@@ -34,10 +34,26 @@ def _elem_bytes(dtype):
     return 8 if dtype == torch.complex64 else 16
 
 
-class CompiledCircuit:
-    """A planned circuit (structure only) bound to one device and dtype."""
+def default_fuse(n, D):
+    """Local modes per fused stage: the largest K in {3, 2} with a D^K tile of
+    at most 4096 amplitudes (stage kernels exist for cutoffs 2..20)."""
+    if n < 2 or D < 2 or D > 20:
+        return 0
+    for K in (3, 2):
+        if K <= n and D**K <= 4096:
+            return K
+    return 0
 
-    def __init__(self, ops, n, D, mixed_in, dtype, device):
+
+class CompiledCircuit:
+    """A planned circuit (structure only) bound to one device and dtype.
+
+    The plan is a list of steps: fused stages (many gates on LDS tiles of
+    ``fuse`` local modes, pure states) or single operations.  Checkpoints and
+    segments are expressed in steps; gradients are indexed by operation.
+    """
+
+    def __init__(self, ops, n, D, mixed_in, dtype, device, fuse=None):
         self.lib = _lib.load()
         self.ops = [(name, tuple(int(m) for m in modes)) for name, modes in ops]
         self.n, self.D, self.mixed_in = int(n), int(D), bool(mixed_in)
@@ -51,8 +67,9 @@ class CompiledCircuit:
             arr[i].mode1 = modes[1] if len(modes) > 1 else -1
             arr[i].pad = 0
         handle = ctypes.c_void_p()
+        self.fuse = default_fuse(self.n, self.D) if fuse is None else int(fuse)
         st = self.lib.sfamd_prog_create(
-            arr, len(self.ops), self.n, self.D, int(self.mixed_in), self.code, ctypes.byref(handle)
+            arr, len(self.ops), self.n, self.D, int(self.mixed_in), self.code, self.fuse, ctypes.byref(handle)
         )
         _lib.check(st, "sfamd_prog_create")
         self.handle = handle
@@ -60,19 +77,24 @@ class CompiledCircuit:
         _lib.check(self.lib.sfamd_prog_info(handle, ctypes.byref(info)), "sfamd_prog_info")
         self.info = info
         self.nops = len(self.ops)
+        self.nsteps = info.nsteps
         self.final_mixed = bool(info.final_mixed)
-        self.state_elems = [self.lib.sfamd_prog_state_elems(handle, i) for i in range(self.nops + 1)]
-        self.mixed_at = []
-        m = self.mixed_in
-        for name, _ in self.ops:
-            self.mixed_at.append(m)
-            if name in ("LOSS", "MIX"):
-                m = True
-        self.mixed_at.append(m)
+        self.state_elems = [self.lib.sfamd_prog_state_elems(handle, i) for i in range(self.nsteps + 1)]
+        full = self.D ** (2 * self.n)
+        self.mixed_at = [e == full and (self.n > 0 and (self.D > 1 or self.mixed_in)) for e in self.state_elems]
+        if self.D == 1:  # sizes cannot tell kets from DMs
+            self.mixed_at = [self.mixed_in] * (self.nsteps + 1)
+            self.mixed_at[-1] = self.final_mixed
+        buf = (ctypes.c_int32 * max(1, self.nops))()
+        self.step_ops = []
+        for i in range(self.nsteps):
+            k = self.lib.sfamd_prog_step_ops(handle, i, buf, self.nops)
+            self.step_ops.append([buf[j] for j in range(k)])
         dev = self.device
         self.table = torch.empty(max(1, info.table_elems), dtype=dtype, device=dev)
         self.ws = torch.empty(max(1, info.ws_elems), dtype=dtype, device=dev)
         self.partial = torch.empty(max(1, info.partial_doubles), dtype=torch.float64, device=dev)
+        self._built = None
 
     def __del__(self):
         h = getattr(self, "handle", None)
@@ -97,17 +119,26 @@ class CompiledCircuit:
     def _stream(self):
         return _lib.stream_ptr(self.device)
 
-    def build_table(self, params):
-        """params: float64 [nops, 2] on the device."""
+    def build_table(self, params, force=True):
+        """Gate tables from params (float64 [nops, 2] on the device).
+
+        With force=False the build is skipped when the table already holds
+        this exact parameter tensor (same object, unmodified), as between the
+        forward and the backward of one evaluation.
+        """
         assert params.dtype == torch.float64 and params.is_contiguous()
         if self.nops == 0:
+            return
+        key = (id(params), params._version)
+        if not force and self._built is not None and self._built[0] is params and self._built[1] == key:
             return
         _lib.call(
             "sfamd_prog_build_table", self.handle, _lib.ptr(params), _lib.ptr(self.table), self._stream()
         )
+        self._built = (params, key)
 
     def forward(self, begin, end, state_in, out=None, saved=None):
-        """Run ops [begin, end) on state_in; saved[j] receives the input of op begin+j."""
+        """Run steps [begin, end) on state_in; saved[j] receives the input of step begin+j."""
         if out is None:
             out = self.new_state(end)
         sv = None
@@ -150,22 +181,24 @@ class CompiledCircuit:
 
     # ---- checkpoint plan
     def segments(self, segment=None, budget=DEFAULT_SAVE_BUDGET):
-        """Split [0, nops) into checkpoint segments.
+        """Split the steps [0, nsteps) into checkpoint segments.
 
-        segment=None keeps every operation input when they fit in `budget`
-        bytes, else uses segments of ~sqrt(nops) operations; an int gives the
-        segment length (e.g. the layer length of a layered circuit).
+        segment=None keeps every step input when they fit in `budget` bytes,
+        else uses segments of ~sqrt(nsteps) steps; an int gives the segment
+        length in steps.  Within a segment the backward recomputes the
+        forward from the segment's checkpoint.
         """
-        if self.nops == 0:
+        ns = self.nsteps
+        if ns == 0:
             return []
         if segment is None:
-            total = sum(self.state_bytes(i) for i in range(self.nops))
+            total = sum(self.state_bytes(i) for i in range(ns))
             if total <= budget:
-                segment = self.nops
+                segment = ns
             else:
-                segment = max(1, int(math.ceil(math.sqrt(self.nops))))
+                segment = max(1, int(math.ceil(math.sqrt(ns))))
         segment = max(1, int(segment))
-        return [(b, min(b + segment, self.nops)) for b in range(0, self.nops, segment)]
+        return [(b, min(b + segment, ns)) for b in range(0, ns, segment)]
 
 
 def _params_row(p):
@@ -220,7 +253,7 @@ class CircuitFunction(torch.autograd.Function):
             return None, g_out, None, None
         ckpts = ctx.saved_tensors
         # the gate table may have been rebuilt by another run of the same plan
-        cc.build_table(ctx.params)
+        cc.build_table(ctx.params, force=False)
         grads = torch.zeros(cc.nops * 2, dtype=torch.float64, device=cc.device)
         lam = g_out.to(cc.dtype).contiguous()
         for si in range(len(ctx.segs) - 1, -1, -1):
@@ -240,9 +273,9 @@ def run_circuit(cc, params, state_in, segment=None):
         return CircuitFunction.apply(params, state_in, cc, segment)
     params = params.detach().contiguous()
     cc.build_table(params)
-    if cc.nops == 0:
+    if cc.nsteps == 0:
         return state_in.clone()
-    return cc.forward(0, cc.nops, state_in.detach().contiguous())
+    return cc.forward(0, cc.nsteps, state_in.detach().contiguous())
 
 
 # --------------------------------------------------------------------- readout
@@ -420,3 +453,143 @@ def gate_matrix(kind, p0, p1, D, dtype, device):
     if kind in ("R", "K"):
         return torch.diag(t[:D])
     return t[: D * D].reshape(D, D)
+
+
+# --------------------------------------------------------------------- fidelity loss
+def overlap(a, b):
+    """<a|b> over all amplitudes (FP64 accumulation); 0-dim complex128 device tensor."""
+    lib = _lib.load()
+    n = a.numel()
+    part = _scratch(lib.sfamd_overlap_partials(n), torch.float64, a.device)
+    out = torch.empty(3, dtype=torch.float64, device=a.device)
+    _lib.call(
+        "sfamd_overlap",
+        _lib.ptr(a.contiguous()),
+        _lib.ptr(b.contiguous()),
+        n,
+        _lib.ptr(part),
+        _lib.ptr(out),
+        _lib.dtype_code(a.dtype),
+        _lib.stream_ptr(a.device),
+    )
+    return torch.complex(out[0], out[1])
+
+
+class FidelityLoss(torch.autograd.Function):
+    """L = 1 - |<target|psi>|^2 and its cotangent -2 <target|psi> target, in one library call."""
+
+    @staticmethod
+    def forward(ctx, psi, target):
+        lib = _lib.load()
+        psi = psi.detach().contiguous()
+        target = target.detach().to(psi.dtype).contiguous()
+        n = psi.numel()
+        part = _scratch(lib.sfamd_overlap_partials(n), torch.float64, psi.device)
+        out = torch.empty(3, dtype=torch.float64, device=psi.device)
+        g = torch.empty_like(psi)
+        _lib.call(
+            "sfamd_fidelity_loss",
+            _lib.ptr(target),
+            _lib.ptr(psi),
+            n,
+            _lib.ptr(part),
+            _lib.ptr(out),
+            _lib.ptr(g),
+            _lib.dtype_code(psi.dtype),
+            _lib.stream_ptr(psi.device),
+        )
+        ctx.save_for_backward(g)
+        return out[2]
+
+    @staticmethod
+    def backward(ctx, gl):
+        (g,) = ctx.saved_tensors
+        return g * gl.to(g.real.dtype), None
+
+
+def fidelity_loss(psi, target):
+    return FidelityLoss.apply(psi, target)
+
+
+class GradEvaluator:
+    """One captured gradient evaluation of L = 1 - |<target|psi(params)>|^2.
+
+    All buffers (parameters, every step input, cotangents, gradients) are
+    allocated once; the gate-table build, the fused forward, the loss and the
+    adjoint sweep are recorded into one HIP graph on a private stream and
+    replayed per call.  Parameters are copied into the static buffer before
+    each replay.  Requires the whole plan's step inputs to fit in memory.
+    """
+
+    def __init__(self, cc, state_in, target, use_graph=True, forward_only=False):
+        lib = _lib.load()
+        self.cc = cc
+        dev = cc.device
+        self.stream = torch.cuda.Stream(device=dev)
+        self.params = torch.zeros((cc.nops, 2), dtype=torch.float64, device=dev)
+        self.state_in = state_in.detach().to(cc.dtype).contiguous().clone()
+        self.target = target.detach().to(cc.dtype).contiguous().clone().reshape(-1)
+        self.saved = [self.state_in] + [cc.new_state(i) for i in range(1, cc.nsteps)]
+        self.psi = cc.new_state(cc.nsteps)
+        self.g_psi = torch.empty_like(self.psi)
+        self.g_in = cc.new_state(0)
+        self.grads = torch.zeros(cc.nops * 2, dtype=torch.float64, device=dev)
+        self.out = torch.zeros(3, dtype=torch.float64, device=dev)
+        self.part = torch.empty(max(1, lib.sfamd_overlap_partials(self.psi.numel())), dtype=torch.float64, device=dev)
+        self.graph = None
+        self.forward_only = forward_only
+        torch.cuda.synchronize(dev)
+        with torch.cuda.stream(self.stream):
+            self._enqueue()  # eager run: warms the kernels, checks every call
+        self.stream.synchronize()
+        if use_graph:
+            sp = ctypes.c_void_p(self.stream.cuda_stream)
+            _lib.call("sfamd_graph_begin", sp)
+            try:
+                with torch.cuda.stream(self.stream):
+                    self._enqueue()
+            finally:
+                g = ctypes.c_void_p()
+                _lib.call("sfamd_graph_end", sp, ctypes.byref(g))
+            self.graph = g
+        self.stream.synchronize()
+
+    def _enqueue(self):
+        cc = self.cc
+        cc.build_table(self.params)
+        cc.forward(0, cc.nsteps, self.state_in, out=self.psi, saved=self.saved[: cc.nsteps])
+        _lib.call(
+            "sfamd_fidelity_loss",
+            _lib.ptr(self.target),
+            _lib.ptr(self.psi),
+            self.psi.numel(),
+            _lib.ptr(self.part),
+            _lib.ptr(self.out),
+            None if self.forward_only else _lib.ptr(self.g_psi),
+            cc.code,
+            _lib.stream_ptr(cc.device),
+        )
+        if not self.forward_only:
+            cc.adjoint(0, cc.nsteps, self.saved[: cc.nsteps], self.g_psi, self.g_in, self.grads)
+
+    def __call__(self, params):
+        """Returns (loss, grads [nops, 2]) as device tensors valid until the next call."""
+        cur = torch.cuda.current_stream(self.cc.device)
+        self.stream.wait_stream(cur)
+        with torch.cuda.stream(self.stream):
+            self.params.copy_(params.detach().reshape(self.cc.nops, 2))
+            if self.graph is not None:
+                _lib.call("sfamd_graph_launch", self.graph, ctypes.c_void_p(self.stream.cuda_stream))
+            else:
+                self._enqueue()
+        cur.wait_stream(self.stream)
+        return self.out[2], self.grads.reshape(self.cc.nops, 2)
+
+    def __del__(self):
+        g = getattr(self, "graph", None)
+        if g is not None:
+            try:
+                _lib.load().sfamd_graph_destroy(g)
+            except Exception:
+                pass
+            self.graph = None

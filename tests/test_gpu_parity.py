@@ -118,7 +118,10 @@ def test_beamsplitter_and_loss_tables(dtype, D):
         f = lambda x: fo.beamsplitter(x, ph, D)  # noqa: E731
         dB = (-f(th + 2 * h) + 8 * f(th + h) - 8 * f(th - h) + f(th - 2 * h)) / (12 * h)
         dBg, _ = _bs_dense_from_blocks(t[nb:], D)
-        assert np.abs(dBg - dB).max() < (1e-8 if dtype == C128 else 1e-5)
+        # relative to the largest derivative entry: the difference quotient's
+        # truncation error grows with the photon number (powers up to 2D - 2)
+        scale = max(1.0, np.abs(dB).max())
+        assert np.abs(dBg - dB).max() < (1e-8 if dtype == C128 and D < 15 else 1e-6) * scale
     g = np.load(os.path.join(GOLDEN, "gates_mpmath.npz"))
     for key in [k for k in g.files if k.startswith("BS") and not k.endswith("_params")]:
         th, ph, Dg = g[key + "_params"]
@@ -389,3 +392,125 @@ def test_mean_photon_gradient_closed_form():
     mean.backward()
     assert abs(float(mean.detach()) - 0.09) < 1e-12
     assert abs(float(r.grad) - 0.6) < 1e-10
+
+
+# ------------------------------------------------------------------ fused stages
+@pytest.mark.parametrize("dtype", [C128, C64])
+@pytest.mark.parametrize("n,D,fuse", [(3, 5, 2), (3, 5, 3), (4, 4, 4), (6, 3, 3), (5, 6, 3), (2, 12, 2)])
+def test_fused_stages_match_unfused_and_oracle(dtype, n, D, fuse):
+    rng = np.random.default_rng(100 + n * D)
+    ops = circuits.layered(n, 2, rng, sd=0.3)
+    flat = np.array([v for _, _, p in ops for v in p], dtype=float)
+    target = torch.as_tensor(circuits.target_state(n, D, 3, rng), dtype=dtype, device=DEV)
+    rows = np.zeros((len(ops), 2))
+    for i, (_, _, p) in enumerate(ops):
+        rows[i, : len(p)] = p
+    ref, _ = fo.run_ops(ops, n, D)
+    out = {}
+    for f in (0, fuse):
+        cc = runtime.CompiledCircuit([(o[0], o[1]) for o in ops], n, D, False, dtype, DEV, fuse=f)
+        if f:
+            assert cc.nsteps < len(ops)
+        params = torch.tensor(rows, dtype=torch.float64, device=DEV, requires_grad=True)
+        vac = torch.zeros([D] * n, dtype=dtype, device=DEV)
+        vac.view(-1)[0] = 1
+        psi = runtime.CircuitFunction.apply(params, vac, cc, None)
+        ov = torch.vdot(target.reshape(-1), psi.reshape(-1))
+        (1 - ov.abs() ** 2).backward()
+        out[f] = (to_np(psi), params.grad.cpu().numpy())
+        assert np.abs(out[f][0] - ref).max() < TOL[dtype] * 10
+    g0, gf = out[0][1], out[fuse][1]
+    rel = np.linalg.norm(g0 - gf) / np.linalg.norm(g0)
+    assert rel < (1e-10 if dtype == C128 else 1e-5), rel
+    del flat
+
+
+def test_fused_gradient_vs_finite_differences_c128():
+    n, D = 4, 5
+    rng = np.random.default_rng(31)
+    ops = circuits.layered(n, 1, rng, sd=0.3)
+    target = circuits.target_state(n, D, 4, rng)
+    flat, g_fd = _fd_grad(ops, n, D, True, target, None)
+    params = torch.tensor(flat, dtype=torch.float64, requires_grad=True)
+    be = HIPFockBackend()
+    be.begin_circuit(n, cutoff_dim=D, dtype=C128, device=DEV)
+    k = 0
+    for name, modes, p in ops:
+        apply(be, name, modes, [params[k + j] for j in range(len(p))])
+        k += len(p)
+    st = be.state()
+    loss = _loss_pure(st.ket(), torch.as_tensor(target, dtype=C128, device=DEV))
+    loss.backward()
+    g = params.grad.numpy()
+    assert np.linalg.norm(g - g_fd) / np.linalg.norm(g_fd) < 1e-8
+
+
+# ------------------------------------------------------------------ captured evaluation
+def test_graph_evaluator_matches_eager_over_replays():
+    """Replays of the captured gradient evaluation equal the eager autograd result
+    for several parameter sets, with eager work and synchronisations between replays."""
+    n, D = 4, 6
+    rng = np.random.default_rng(41)
+    ops = circuits.layered(n, 2, rng, sd=0.3)
+    target = torch.as_tensor(circuits.target_state(n, D, 4, rng), dtype=C64, device=DEV)
+    cc = runtime.CompiledCircuit([(o[0], o[1]) for o in ops], n, D, False, C64, DEV)
+    vac = torch.zeros([D] * n, dtype=C64, device=DEV)
+    vac.view(-1)[0] = 1
+    ev = runtime.GradEvaluator(cc, vac, target, use_graph=True)
+    for trial in range(4):
+        rows = np.zeros((len(ops), 2))
+        for i, (_, _, p) in enumerate(ops):
+            rows[i, : len(p)] = np.array(p) * (1 + 0.1 * trial)
+        params = torch.tensor(rows, dtype=torch.float64, device=DEV)
+        loss_g, grads_g = ev(params)
+        loss_g, grads_g = float(loss_g), grads_g.clone()
+        p_leaf = params.clone().requires_grad_(True)
+        psi = runtime.CircuitFunction.apply(p_leaf, vac, cc, None)
+        loss = runtime.fidelity_loss(psi, target)
+        loss.backward()
+        torch.cuda.synchronize()
+        assert abs(loss_g - float(loss.detach())) < 1e-12
+        assert torch.equal(grads_g, p_leaf.grad)
+        ov = runtime.overlap(target, psi.detach())
+        ref = torch.vdot(target.reshape(-1).to(torch.complex128), psi.detach().reshape(-1).to(torch.complex128))
+        assert abs(complex(ov) - complex(ref)) < 1e-6
+
+
+def test_fidelity_loss_gradient_matches_torch():
+    rng = np.random.default_rng(3)
+    psi = torch.tensor(rng.normal(size=300) + 1j * rng.normal(size=300), dtype=C128, device=DEV, requires_grad=True)
+    t = torch.tensor(rng.normal(size=300) + 1j * rng.normal(size=300), dtype=C128, device=DEV)
+    l1 = runtime.fidelity_loss(psi, t)
+    (g1,) = torch.autograd.grad(l1, psi)
+    ov = torch.vdot(t, psi)
+    l2 = 1 - ov.real**2 - ov.imag**2
+    (g2,) = torch.autograd.grad(l2, psi)
+    assert abs(float(l1) - float(l2)) < 1e-9 * abs(float(l2))
+    assert (g1 - g2).abs().max() < 1e-9 * g2.abs().max()
+
+
+def test_tf_path_comparator_gradient_matches_adjoint():
+    """The TF-path restatement (dense einsums + autograd) and the hand-written
+    adjoint give the same gradient (complex128)."""
+    from strawberryfields_amd import tfpath
+
+    n, D = 3, 6
+    rng = np.random.default_rng(51)
+    ops = circuits.layered(n, 1, rng, sd=0.3)
+    target = torch.as_tensor(circuits.target_state(n, D, 4, rng), dtype=C128, device=DEV)
+    rows = np.zeros((len(ops), 2))
+    for i, (_, _, p) in enumerate(ops):
+        rows[i, : len(p)] = p
+    p1 = torch.tensor(rows, dtype=torch.float64, device=DEV, requires_grad=True)
+    psi = tfpath.TFPathCircuit([(o[0], o[1]) for o in ops], n, D, C128, DEV).forward(p1)
+    ov = torch.vdot(target.reshape(-1), psi.reshape(-1))
+    (1 - ov.abs() ** 2).backward()
+    cc = runtime.CompiledCircuit([(o[0], o[1]) for o in ops], n, D, False, C128, DEV)
+    p2 = torch.tensor(rows, dtype=torch.float64, device=DEV, requires_grad=True)
+    vac = torch.zeros([D] * n, dtype=C128, device=DEV)
+    vac.view(-1)[0] = 1
+    psi2 = runtime.CircuitFunction.apply(p2, vac, cc, None)
+    runtime.fidelity_loss(psi2, target).backward()
+    assert (psi.detach() - psi2.detach()).abs().max() < 1e-12
+    g1, g2 = p1.grad, p2.grad
+    assert float((g1 - g2).norm() / g2.norm()) < 1e-10

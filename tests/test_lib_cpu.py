@@ -52,13 +52,13 @@ def test_gate_table_sizes():
     assert lib.sfamd_gate_table_size(_lib.OP_SQUEEZE, 65) == -1
 
 
-def _create(ops, n, D, mixed=0, dtype=_lib.C64):
+def _create(ops, n, D, mixed=0, dtype=_lib.C64, fuse=0):
     lib = _lib.load()
     arr = (_lib.OpDesc * max(1, len(ops)))()
     for i, (k, a, b) in enumerate(ops):
         arr[i].kind, arr[i].mode0, arr[i].mode1 = k, a, b
     h = ctypes.c_void_p()
-    st = lib.sfamd_prog_create(arr, len(ops), n, D, mixed, dtype, ctypes.byref(h))
+    st = lib.sfamd_prog_create(arr, len(ops), n, D, mixed, dtype, fuse, ctypes.byref(h))
     return st, h
 
 
@@ -75,7 +75,7 @@ def test_program_planning_and_validation():
     assert st == 0
     info = _lib.ProgInfo()
     assert lib.sfamd_prog_info(h, ctypes.byref(info)) == 0
-    assert info.nops == 4 and info.final_mixed == 1
+    assert info.nops == 4 and info.final_mixed == 1 and info.nsteps == 4
     assert info.final_elems == D ** (2 * n)
     assert lib.sfamd_prog_state_elems(h, 0) == D**n
     assert lib.sfamd_prog_state_elems(h, 2) == D**n
@@ -180,3 +180,53 @@ def test_workload_generators():
     assert len(c5["ops"]) == 144 and sum(1 for o in c5["ops"] if o[0] == "LOSS") == 18
     # same seed -> same parameters
     assert circuits.config3()["ops"] == c3["ops"]
+
+
+def _plan(ops, n, D, fuse, dtype=_lib.C64):
+    lib = _lib.load()
+    codes = [(_lib.OP_KINDS[name], m[0], m[1] if len(m) > 1 else -1) for name, m, _ in ops]
+    st, h = _create(codes, n, D, dtype=dtype, fuse=fuse)
+    assert st == 0
+    info = _lib.ProgInfo()
+    lib.sfamd_prog_info(h, ctypes.byref(info))
+    buf = (ctypes.c_int32 * len(ops))()
+    steps = []
+    for i in range(info.nsteps):
+        k = lib.sfamd_prog_step_ops(h, i, buf, len(ops))
+        steps.append([buf[j] for j in range(k)])
+    lib.sfamd_prog_destroy(h)
+    return steps
+
+
+@pytest.mark.parametrize("fuse", [0, 2, 3, 4])
+def test_stage_planner_keeps_every_gate_and_mode_order(fuse):
+    """The fused plan covers each op once, keeps per-mode order and local sets."""
+    for cfg, dtype in [(circuits.config3(), _lib.C64), (circuits.config3(), _lib.C128), (circuits.config2(), _lib.C64)]:
+        ops, n, D = cfg["ops"], cfg["n"], cfg["D"]
+        steps = _plan(ops, n, D, fuse, dtype)
+        flat = [i for st in steps for i in st]
+        assert sorted(flat) == list(range(len(ops)))
+        for m in range(n):  # gates on one mode keep their program order
+            order = [i for i in flat if m in ops[i][1]]
+            assert order == sorted(order)
+        for st in steps:
+            modes = set(mm for i in st for mm in ops[i][1])
+            assert len(modes) <= max(fuse, 2)
+        if fuse == 0:
+            assert len(steps) == len(ops)
+    c3 = circuits.config3()
+    if fuse == 3:
+        assert len(_plan(c3["ops"], 6, 8, 3)) < 60
+
+
+def test_stage_planner_leaves_mixed_ops_unfused():
+    rng = np.random.default_rng(1)
+    ops = circuits.layered(3, 1, rng, loss=True)
+    steps = _plan(ops, 3, 6, 3)
+    first_loss = min(i for i, o in enumerate(ops) if o[0] == "LOSS")
+    for st in steps:
+        if len(st) > 1:
+            assert all(i < first_loss for i in st)
+    # every op from the first loss on is its own step
+    tail = [st for st in steps if st[0] >= first_loss]
+    assert all(len(st) == 1 for st in tail)
