@@ -45,7 +45,216 @@ def parse():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--fuse", type=int, default=None, help="local modes per fused stage (0 = one kernel per gate)")
+    ap.add_argument(
+        "--workload",
+        default="c3",
+        choices=["c3", "c4", "c5", "hbm"],
+        help="c3: 6-mode ket gradient (default line); c4: 256 uneven circuits in one graph launch; "
+        "c5: 3-mode cutoff-20 density-matrix gradient with loss and layer checkpoints; "
+        "hbm: per-kernel HBM rates on a 2 GB ket and the 512 MB DM",
+    )
     return ap.parse_args()
+
+
+def _event_us(fn, reps, stream=None):
+    import torch
+
+    fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    st = stream or torch.cuda.current_stream()
+    e0.record(st)
+    for _ in range(reps):
+        fn()
+    e1.record(st)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) * 1e3 / reps
+
+
+def kernel_rates(n, D, mixed, dtype, reps=10):
+    """Average launch time and algorithmic GB/s of each operation kind on one
+    state: forward 2 S per operation (two passes on a DM for D, S, BS), adjoint
+    4 S (forward state, cotangent in, cotangent out, + recompute on a DM)."""
+    import torch
+
+    from strawberryfields_amd import runtime
+
+    dev = torch.device("cuda:0")
+    esz = 8 if dtype == torch.complex64 else 16
+    S = D ** (2 * n if mixed else n) * esz
+    shape = [D] * (2 * n if mixed else n)
+    st = torch.randn(shape, dtype=dtype, device=dev)
+    out = torch.empty_like(st)
+    lam = torch.randn_like(st)
+    gin = torch.empty_like(st)
+    kinds = [("D", (0,)), ("D", (n - 1,)), ("S", (n // 2,)), ("R", (0,)), ("K", (n - 1,)), ("BS", (0, 1)), ("BS", (n - 2, n - 1))]
+    if mixed:
+        kinds += [("LOSS", (0,)), ("LOSS", (n - 1,))]
+    res = {}
+    for kind, modes in kinds:
+        cc = runtime.CompiledCircuit([(kind, modes)], n, D, mixed, dtype, dev, fuse=0)
+        cc.build_table(torch.tensor([[0.3, 0.2] if kind != "LOSS" else [0.8, 0.0]], dtype=torch.float64, device=dev))
+        grads = torch.zeros(2, dtype=torch.float64, device=dev)
+        tf = _event_us(lambda: cc.forward(0, 1, st, out=out), reps)
+        ta = _event_us(lambda: cc.adjoint(0, 1, [st], lam, gin, grads), reps)
+        res["%s%s" % (kind, list(modes))] = {
+            "fwd_us": round(tf, 1),
+            "fwd_GBs": round(2 * S / tf / 1e3, 1),
+            "adj_us": round(ta, 1),
+            "adj_GBs": round(4 * S / ta / 1e3, 1),
+        }
+    return S, res
+
+
+def run_c5(args):
+    """3-mode DM at cutoff 20 (64M complex64, 512 MB), 6 layers with loss:
+    forward + adjoint gradient of sum_n p(n) (n1+n2+n3-3)^2, checkpoints at
+    layer boundaries (24 steps), the rest recomputed in the backward."""
+    import numpy as np
+    import torch
+
+    from strawberryfields_amd import circuits, runtime
+
+    dev = torch.device("cuda:0")
+    c5 = circuits.config5()
+    n, D, ops = c5["n"], c5["D"], c5["ops"]
+    dtype = torch.complex64
+    cc = runtime.CompiledCircuit([(o[0], o[1]) for o in ops], n, D, True, dtype, dev)
+    rows = np.zeros((len(ops), 2))
+    for i, (_, _, p) in enumerate(ops):
+        rows[i, : len(p)] = p
+    params = torch.tensor(rows, dtype=torch.float64, device=dev, requires_grad=True)
+    rho0 = torch.zeros([D] * (2 * n), dtype=dtype, device=dev)
+    rho0.view(-1)[0] = 1.0
+    idx = torch.arange(D, device=dev, dtype=torch.float64)
+    w = (idx[:, None, None] + idx[None, :, None] + idx[None, None, :] - 3.0) ** 2
+
+    def step():
+        params.grad = None
+        rho = runtime.CircuitFunction.apply(params, rho0, cc, c5["layer_len"])
+        probs = runtime.all_fock_probs(rho, n, D, True)
+        loss = (probs.to(torch.float64) * w).sum()
+        loss.backward()
+        return loss
+
+    step()
+    torch.cuda.synchronize()
+    reps = max(2, min(args.steps, 5))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        loss = step()
+    torch.cuda.synchronize()
+    t_grad = (time.perf_counter() - t0) / reps
+    with torch.no_grad():
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            rho = runtime.run_circuit(cc, params.detach(), rho0)
+        torch.cuda.synchronize()
+        t_fwd = (time.perf_counter() - t0) / reps
+    tr = float(runtime.norm2(rho, n, D, True))
+    herm = float((rho.reshape(D**n, D**n) - rho.reshape(D**n, D**n).conj().T).abs().max())
+    S, rates = kernel_rates(n, D, True, dtype)
+    nops = len(ops)
+    fwd_alg = 2 * S * nops
+    grad_alg = 6 * S * nops
+    line = {
+        "metric": "grad-evals/s (fwd+adjoint), 3-mode cutoff-20 density matrix with loss",
+        "value": round(1.0 / t_grad, 3),
+        "unit": "grad-evals/s",
+        "n_gpus": 1,
+        "steps": reps,
+        "ms_per_step": round(t_grad * 1e3, 2),
+        "fwd_ms": round(t_fwd * 1e3, 2),
+        "higher_is_better": True,
+        "dtype": "complex64",
+        "data": "synthetic",
+        "config": {"workload": "3-mode DM, cutoff 20, 6 layers with loss (144 ops), layer checkpoints", "state_MB": S / 2**20},
+        "loss": float(loss.detach()),
+        "grad_norm": float(params.grad.norm()),
+        "trace": tr,
+        "hermiticity_err": herm,
+        "fwd_algorithmic_GBs": round(fwd_alg / t_fwd / 1e9, 1),
+        "grad_algorithmic_GBs": round(grad_alg / t_grad / 1e9, 1),
+        "kernels": rates,
+    }
+    print(json.dumps(line))
+
+
+def run_c4(args):
+    """256 independent pure circuits (n in 2..5, cutoff 6..20, D^n <= 2^20,
+    2-6 layers): forward + adjoint gradient of each circuit's total mean photon
+    number, all replayed from one HIP graph; stage tiles from the autotuner."""
+    import numpy as np
+    import torch
+
+    from strawberryfields_amd import circuits, runtime
+
+    dev = torch.device("cuda:0")
+    c4 = circuits.config4()
+    circs = c4["circuits"]
+    tuner = runtime.Autotuner(path=os.environ.get("SFAMD_AUTOTUNE_CACHE"))
+    t0 = time.perf_counter()
+    be = runtime.BatchEvaluator(circs, torch.complex64, dev, nstreams=8, tuner=tuner)
+    t_setup = time.perf_counter() - t0
+    plist = []
+    for c in circs:
+        rows = np.zeros((len(c["ops"]), 2))
+        for i, (_, _, p) in enumerate(c["ops"]):
+            rows[i, : len(p)] = p
+        plist.append(torch.tensor(rows, dtype=torch.float64, device=dev))
+    for _ in range(2):
+        be(plist)
+    torch.cuda.synchronize()
+    reps = max(3, min(args.steps, 20))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        losses, grads = be(plist)
+    torch.cuda.synchronize()
+    t_eval = (time.perf_counter() - t0) / reps
+    amps = sum(c["D"] ** c["n"] for c in circs)
+    alg = sum(6 * 8 * c["D"] ** c["n"] * len(c["ops"]) for c in circs)
+    fuse_hist = {}
+    for it in be.items:
+        fuse_hist[str(it["cc"].fuse)] = fuse_hist.get(str(it["cc"].fuse), 0) + 1
+    print(
+        json.dumps(
+            {
+                "metric": "batch grad-evals/s (256 circuits per evaluation, one graph launch)",
+                "value": round(1.0 / t_eval, 3),
+                "unit": "batch-evals/s",
+                "circuit_grad_evals_per_s": round(len(circs) / t_eval, 1),
+                "ms_per_batch": round(t_eval * 1e3, 3),
+                "n_gpus": 1,
+                "steps": reps,
+                "dtype": "complex64",
+                "data": "synthetic",
+                "config": {"workload": "256 uneven pure circuits, loss = total mean photon number", "amplitudes": amps},
+                "grad_algorithmic_GBs": round(alg / t_eval / 1e9, 1),
+                "setup_s_incl_autotune": round(t_setup, 2),
+                "autotuned_tiles": fuse_hist,
+                "mean_loss": float(losses.mean()),
+            }
+        )
+    )
+
+
+def run_hbm(args):
+    """Per-operation HBM rates on a 2 GB pure state (7 modes, cutoff 16) and on
+    the 512 MB density matrix (3 modes, cutoff 20), complex64."""
+    import torch
+
+    S1, r1 = kernel_rates(7, 16, False, torch.complex64)
+    S2, r2 = kernel_rates(3, 20, True, torch.complex64)
+    print(
+        json.dumps(
+            {
+                "metric": "algorithmic HBM GB/s per operation (peak %d GB/s)" % HBM_PEAK_GBS,
+                "ket_7modes_cutoff16": {"state_MB": S1 / 2**20, "ops": r1},
+                "dm_3modes_cutoff20": {"state_MB": S2 / 2**20, "ops": r2},
+            }
+        )
+    )
 
 
 def main():
@@ -65,6 +274,12 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    if args.workload == "c5":
+        return run_c5(args)
+    if args.workload == "c4":
+        return run_c4(args)
+    if args.workload == "hbm":
+        return run_hbm(args)
     c3 = circuits.config3()
     n, D, ops = c3["n"], c3["D"], c3["ops"]
     dtype = torch.complex64

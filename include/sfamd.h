@@ -167,6 +167,14 @@ int sfamd_fidelity_loss(const void* target, const void* psi, int64_t n_elems,
                         double* partial, double* out_dev, void* g_psi,
                         int dtype, void* stream);
 
+/* Total mean photon number of a ket, L = sum_I |psi_I|^2 sum_m I_m (FP64,
+ * out_dev[0]), and if g_psi != NULL its cotangent g = 2 (sum_m I_m) psi.
+ * (BaseFockState.mean_photon / number_expectation summed over modes,
+ * states.py:657-663,968-974.)  `partial` holds sfamd_reduce_partials(n, D, 0)
+ * doubles. */
+int sfamd_number_loss(const void* psi, void* g_psi, int n, int D, double* partial,
+                      double* out_dev, int dtype, void* stream);
+
 /* ---- recorded circuits (forward, adjoint, graphs) ----------------------- */
 typedef struct {
   int32_t kind;   /* SFAMD_OP_*                                              */

@@ -81,6 +81,7 @@ SIGNATURES = {
     "sfamd_overlap_partials": (_I64, [_I64]),
     "sfamd_overlap": (_I, [_P, _P, _I64, _P, _P, _I, _P]),
     "sfamd_fidelity_loss": (_I, [_P, _P, _I64, _P, _P, _P, _I, _P]),
+    "sfamd_number_loss": (_I, [_P, _P, _I, _I, _P, _P, _I, _P]),
     "sfamd_prog_create": (_I, [ctypes.POINTER(OpDesc), _I, _I, _I, _I, _I, _I, ctypes.POINTER(_P)]),
     "sfamd_prog_info": (_I, [_P, ctypes.POINTER(ProgInfo)]),
     "sfamd_prog_state_elems": (_I64, [_P, _I]),

@@ -624,6 +624,28 @@ k_fidelity_grad(const C<T>* __restrict__ t, const double* __restrict__ ov, C<T>*
   }
 }
 
+// L = sum_I p(I) * (sum_m I_m) (total mean photon number of a ket) and its
+// cotangent g = 2 (sum_m I_m) psi; per-block FP64 partials
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_number_loss(const C<T>* __restrict__ psi, C<T>* __restrict__ g, int n, int D, int64_t N, double* partial) {
+  __shared__ double sh[8];
+  double v = 0.0;
+  GRID_STRIDE(e, N) {
+    int64_t r = e;
+    int tot = 0;
+    for (int m = 0; m < n; ++m) {
+      tot += (int)(r % D);
+      r /= D;
+    }
+    const C<T> x = psi[e];
+    v += (double)tot * ((double)x.re * x.re + (double)x.im * x.im);
+    if (g) g[e] = mk<T>((T)(2 * tot) * x.re, (T)(2 * tot) * x.im);
+  }
+  const double s = block_sum(v, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
 // ------------------------------------------------------------------ helpers
 #define CP(T, p) ((const C<T>*)(p))
 #define MP(T, p) ((C<T>*)(p))
@@ -965,5 +987,22 @@ extern "C" int sfamd_fidelity_loss(const void* target, const void* psi, int64_t 
   else
     hipLaunchKernelGGL(k_fidelity_grad<double>, dim3(grid_for(n_elems)), dim3(kBlock), 0, s, CP(double, target),
                        out_dev, MP(double, g_psi), n_elems);
+  return (int)hipGetLastError();
+}
+
+extern "C" int sfamd_number_loss(const void* psi, void* g_psi, int n, int D, double* partial, double* out_dev,
+                                 int dtype, void* stream) {
+  CHECK_ARG(psi && partial && out_dev && n >= 1 && D >= 1);
+  hipStream_t s = (hipStream_t)stream;
+  const int64_t N = ipow(D, n);
+  const int g = (int)reduce_grid(N);
+  if (dtype == SFAMD_C64)
+    hipLaunchKernelGGL(k_number_loss<float>, dim3(g), dim3(kBlock), 0, s, CP(float, psi), MP(float, g_psi), n, D, N,
+                       partial);
+  else
+    hipLaunchKernelGGL(k_number_loss<double>, dim3(g), dim3(kBlock), 0, s, CP(double, psi), MP(double, g_psi), n, D,
+                       N, partial);
+  HIP_RET(hipGetLastError());
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, s, partial, g, out_dev);
   return (int)hipGetLastError();
 }

@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "common.h"
+#include "fiber.h"
 #include "fused.h"
 
 using namespace sfamd;
@@ -49,6 +50,47 @@ int validate_op(int kind, int m0, int m1, int n, int D) {
   return 0;
 }
 
+// Routing between element-wise and per-fiber kernels: per-fiber when a kernel
+// is compiled for D and the innermost non-gate run is long enough to coalesce.
+bool use_fib(int D, int64_t run) { return fiber_supported(D) && run >= kFiberMinRun; }
+
+int dense_pass(const void* in, void* out, int64_t N, int D, int64_t B, const void* U, int conj, int parity, int dtype,
+               hipStream_t s) {
+  if (use_fib(D, B)) return axis_dense_fib(in, out, N, D, B, U, conj, parity, dtype, s);
+  return axis_dense(in, out, N, D, B, U, conj, dtype, s);
+}
+
+int dense_adj_pass(const void* psi, const void* lo, void* li, int64_t N, int D, int64_t B, const void* U,
+                   const void* dU0, const void* dU1, int conj, int parity, double* pb, int nwg, int zp1, int dtype,
+                   hipStream_t s) {
+  if (use_fib(D, B)) return axis_dense_adj_fib(psi, lo, li, N, D, B, U, dU0, dU1, conj, parity, pb, nwg, zp1, dtype, s);
+  return axis_dense_adj(psi, lo, li, N, D, B, U, dU0, dU1, conj, pb, nwg, zp1, dtype, s);
+}
+
+int bs_pass(const void* in, void* out, int64_t N, int D, int64_t sp, int64_t sq, const void* g, int conj, int dtype,
+            hipStream_t s) {
+  if (use_fib(D, sp < sq ? sp : sq)) return pair_bs_fib(in, out, N, D, sp, sq, g, conj, dtype, s);
+  return pair_bs(in, out, N, D, sp, sq, g, conj, dtype, s);
+}
+
+int bs_adj_pass(const void* psi, const void* lo, void* li, int64_t N, int D, int64_t sp, int64_t sq, const void* g,
+                const void* dB, int conj, double* pb, int nwg, int zp1, int dtype, hipStream_t s) {
+  if (use_fib(D, sp < sq ? sp : sq)) return pair_bs_adj_fib(psi, lo, li, N, D, sp, sq, g, dB, conj, pb, nwg, zp1, dtype, s);
+  return pair_bs_adj(psi, lo, li, N, D, sp, sq, g, dB, conj, pb, nwg, zp1, dtype, s);
+}
+
+int loss_pass(const void* in, void* out, int64_t N, int D, int64_t sb, int64_t sk, const void* c, int dtype,
+              hipStream_t s) {
+  if (use_fib(D, sk)) return loss_fib(in, out, N, D, sk, c, dtype, s);
+  return loss_fwd(in, out, N, D, sb, sk, c, dtype, s);
+}
+
+int loss_adj_pass(const void* psi, const void* lo, void* li, int64_t N, int D, int64_t sb, int64_t sk, const void* c,
+                  const void* dc, double* pb, int nwg, int zp1, int dtype, hipStream_t s) {
+  if (use_fib(D, sk)) return loss_adj_fib(psi, lo, li, N, D, sk, c, dc, pb, nwg, zp1, dtype, s);
+  return loss_adj(psi, lo, li, N, D, sb, sk, c, dc, pb, nwg, zp1, dtype, s);
+}
+
 // forward of one operation: in -> out (tmp: scratch state for two-pass ops)
 int apply_op(const OpPlan& op, int n, int D, const void* table, const void* in, void* out, void* tmp, int dtype,
              hipStream_t s) {
@@ -61,20 +103,22 @@ int apply_op(const OpPlan& op, int n, int D, const void* table, const void* in, 
         return pair_diag(in, out, NM, D, bra_stride_dm(n, D, op.m0), ket_stride_dm(n, D, op.m0), g, dtype, s);
       return axis_diag(in, out, NK, D, ket_stride(n, D, op.m0), g, 0, dtype, s);
     case SFAMD_OP_DISPLACEMENT:
-    case SFAMD_OP_SQUEEZE:
+    case SFAMD_OP_SQUEEZE: {
+      const int par = op.kind == SFAMD_OP_SQUEEZE;
       if (op.mixed_in) {
         if (!tmp) return SFAMD_E_WORKSPACE;
-        RET_IF(axis_dense(in, tmp, NM, D, bra_stride_dm(n, D, op.m0), g, 0, dtype, s));
-        return axis_dense(tmp, out, NM, D, ket_stride_dm(n, D, op.m0), g, 1, dtype, s);
+        RET_IF(dense_pass(in, tmp, NM, D, bra_stride_dm(n, D, op.m0), g, 0, par, dtype, s));
+        return dense_pass(tmp, out, NM, D, ket_stride_dm(n, D, op.m0), g, 1, par, dtype, s);
       }
-      return axis_dense(in, out, NK, D, ket_stride(n, D, op.m0), g, 0, dtype, s);
+      return dense_pass(in, out, NK, D, ket_stride(n, D, op.m0), g, 0, par, dtype, s);
+    }
     case SFAMD_OP_BEAMSPLITTER:
       if (op.mixed_in) {
         if (!tmp) return SFAMD_E_WORKSPACE;
-        RET_IF(pair_bs(in, tmp, NM, D, bra_stride_dm(n, D, op.m0), bra_stride_dm(n, D, op.m1), g, 0, dtype, s));
-        return pair_bs(tmp, out, NM, D, ket_stride_dm(n, D, op.m0), ket_stride_dm(n, D, op.m1), g, 1, dtype, s);
+        RET_IF(bs_pass(in, tmp, NM, D, bra_stride_dm(n, D, op.m0), bra_stride_dm(n, D, op.m1), g, 0, dtype, s));
+        return bs_pass(tmp, out, NM, D, ket_stride_dm(n, D, op.m0), ket_stride_dm(n, D, op.m1), g, 1, dtype, s);
       }
-      return pair_bs(in, out, NK, D, ket_stride(n, D, op.m0), ket_stride(n, D, op.m1), g, 0, dtype, s);
+      return bs_pass(in, out, NK, D, ket_stride(n, D, op.m0), ket_stride(n, D, op.m1), g, 0, dtype, s);
     case SFAMD_OP_LOSS: {
       const void* src = in;
       if (op.mix_first) {
@@ -82,7 +126,7 @@ int apply_op(const OpPlan& op, int n, int D, const void* table, const void* in, 
         RET_IF(mix_fwd(in, tmp, n, D, dtype, s));
         src = tmp;
       }
-      return loss_fwd(src, out, NM, D, bra_stride_dm(n, D, op.m0), ket_stride_dm(n, D, op.m0), g, dtype, s);
+      return loss_pass(src, out, NM, D, bra_stride_dm(n, D, op.m0), ket_stride_dm(n, D, op.m0), g, dtype, s);
     }
     case SFAMD_OP_MIX:
       return mix_fwd(in, out, n, D, dtype, s);
@@ -112,25 +156,26 @@ int adjoint_op(const OpPlan& op, int n, int D, const void* table, const void* ps
       const int64_t DD = (int64_t)D * D;
       const void* dU0 = cptr(g, DD, dtype);
       const void* dU1 = cptr(g, 2 * DD, dtype);
+      const int par = op.kind == SFAMD_OP_SQUEEZE;
       if (op.mixed_in) {
         const int64_t sb = bra_stride_dm(n, D, op.m0), sk = ket_stride_dm(n, D, op.m0);
         // rho1 = U rho (bra side) recomputed; then the ket-side pass, then the bra-side pass
-        RET_IF(axis_dense(psi, t1, NM, D, sb, g, 0, dtype, s));
-        RET_IF(axis_dense_adj(t1, lo, t2, NM, D, sk, g, dU0, dU1, 1, pass1, nwg, 0, dtype, s));
-        return axis_dense_adj(psi, t2, li, NM, D, sb, g, dU0, dU1, 0, pass0, nwg, 0, dtype, s);
+        RET_IF(dense_pass(psi, t1, NM, D, sb, g, 0, par, dtype, s));
+        RET_IF(dense_adj_pass(t1, lo, t2, NM, D, sk, g, dU0, dU1, 1, par, pass1, nwg, 0, dtype, s));
+        return dense_adj_pass(psi, t2, li, NM, D, sb, g, dU0, dU1, 0, par, pass0, nwg, 0, dtype, s);
       }
-      return axis_dense_adj(psi, lo, li, NK, D, ket_stride(n, D, op.m0), g, dU0, dU1, 0, pass0, nwg, 1, dtype, s);
+      return dense_adj_pass(psi, lo, li, NK, D, ket_stride(n, D, op.m0), g, dU0, dU1, 0, par, pass0, nwg, 1, dtype, s);
     }
     case SFAMD_OP_BEAMSPLITTER: {
       const void* dB = cptr(g, bs_nb(D), dtype);
       if (op.mixed_in) {
         const int64_t bp = bra_stride_dm(n, D, op.m0), bq = bra_stride_dm(n, D, op.m1);
         const int64_t kp = ket_stride_dm(n, D, op.m0), kq = ket_stride_dm(n, D, op.m1);
-        RET_IF(pair_bs(psi, t1, NM, D, bp, bq, g, 0, dtype, s));
-        RET_IF(pair_bs_adj(t1, lo, t2, NM, D, kp, kq, g, dB, 1, pass1, nwg, 0, dtype, s));
-        return pair_bs_adj(psi, t2, li, NM, D, bp, bq, g, dB, 0, pass0, nwg, 0, dtype, s);
+        RET_IF(bs_pass(psi, t1, NM, D, bp, bq, g, 0, dtype, s));
+        RET_IF(bs_adj_pass(t1, lo, t2, NM, D, kp, kq, g, dB, 1, pass1, nwg, 0, dtype, s));
+        return bs_adj_pass(psi, t2, li, NM, D, bp, bq, g, dB, 0, pass0, nwg, 0, dtype, s);
       }
-      return pair_bs_adj(psi, lo, li, NK, D, ket_stride(n, D, op.m0), ket_stride(n, D, op.m1), g, dB, 0, pass0, nwg,
+      return bs_adj_pass(psi, lo, li, NK, D, ket_stride(n, D, op.m0), ket_stride(n, D, op.m1), g, dB, 0, pass0, nwg,
                          1, dtype, s);
     }
     case SFAMD_OP_LOSS: {
@@ -140,10 +185,10 @@ int adjoint_op(const OpPlan& op, int n, int D, const void* table, const void* ps
       const int64_t sb = bra_stride_dm(n, D, op.m0), sk = ket_stride_dm(n, D, op.m0);
       if (op.mix_first) {
         RET_IF(mix_fwd(psi, t1, n, D, dtype, s));
-        RET_IF(loss_adj(t1, lo, t2, NM, D, sb, sk, c, dc, pass0, nwg, 1, dtype, s));
+        RET_IF(loss_adj_pass(t1, lo, t2, NM, D, sb, sk, c, dc, pass0, nwg, 1, dtype, s));
         return mix_adj(psi, t2, li, n, D, dtype, s);
       }
-      return loss_adj(psi, lo, li, NM, D, sb, sk, c, dc, pass0, nwg, 1, dtype, s);
+      return loss_adj_pass(psi, lo, li, NM, D, sb, sk, c, dc, pass0, nwg, 1, dtype, s);
     }
     case SFAMD_OP_MIX:
       RET_IF(fill_zero(pb, 4 * (int64_t)nwg * sizeof(double), s));

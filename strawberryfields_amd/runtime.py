@@ -593,3 +593,222 @@ class GradEvaluator:
             except Exception:
                 pass
             self.graph = None
+
+
+# --------------------------------------------------------------------- autotuning
+def _structure_key(ops, n, D, mixed, dtype):
+    import hashlib
+
+    h = hashlib.sha1(repr([(name, tuple(m)) for name, m in ops]).encode()).hexdigest()[:16]
+    return "%d:%d:%d:%s:%s" % (n, D, int(mixed), str(dtype).split(".")[-1], h)
+
+
+class Autotuner:
+    """Measures the stage tile (local modes K per fused stage, 0 = one kernel
+    per gate) for a circuit structure and caches the fastest.
+
+    Candidates are the K whose stage kernels exist for the cutoff; each is
+    timed over a forward + adjoint sweep with HIP events on random parameters.
+    The choice is cached in memory and, when `path` is set (default
+    $SFAMD_AUTOTUNE_CACHE), in a JSON file keyed by (modes, cutoff, mixed,
+    dtype, structure hash).
+    """
+
+    def __init__(self, path=None, reps=3):
+        import json
+        import os
+
+        self.path = path if path is not None else os.environ.get("SFAMD_AUTOTUNE_CACHE")
+        self.reps = reps
+        self.cache = {}
+        if self.path and os.path.exists(self.path):
+            try:
+                with open(self.path) as f:
+                    self.cache = json.load(f)
+            except (OSError, ValueError):
+                self.cache = {}
+
+    def _save(self):
+        import json
+
+        if self.path:
+            try:
+                with open(self.path, "w") as f:
+                    json.dump(self.cache, f, indent=0, sort_keys=True)
+            except OSError:
+                pass
+
+    @staticmethod
+    def candidates(n, D, mixed):
+        if mixed:
+            return [0]
+        out = [0]
+        for K in (2, 3, 4):
+            if K <= n and 2 <= D <= 20 and D**K <= 4096:
+                out.append(K)
+        return out
+
+    def best(self, ops, n, D, mixed, dtype, device):
+        key = _structure_key(ops, n, D, mixed, dtype)
+        if key in self.cache:
+            return self.cache[key]["fuse"]
+        cands = self.candidates(n, D, mixed)
+        if len(cands) == 1:
+            self.cache[key] = {"fuse": cands[0], "us": {}}
+            return cands[0]
+        dev = torch.device(device)
+        gen = torch.Generator(device="cpu").manual_seed(0)
+        params = (torch.rand((len(ops), 2), generator=gen, dtype=torch.float64) * 0.3).to(dev)
+        st = torch.zeros([D] * (2 * n if mixed else n), dtype=dtype, device=dev)
+        st.view(-1)[0] = 1.0
+        times = {}
+        for K in cands:
+            cc = CompiledCircuit(ops, n, D, mixed, dtype, dev, fuse=K)
+            if K and cc.nsteps == len(ops):
+                continue  # no stage fits this tile
+            cc.build_table(params)
+            saved = [st] + [cc.new_state(i) for i in range(1, cc.nsteps)]
+            out = cc.new_state(cc.nsteps)
+            lam = torch.randn_like(out)
+            gin = cc.new_state(0)
+            grads = torch.zeros(2 * len(ops), dtype=torch.float64, device=dev)
+
+            def run():
+                cc.forward(0, cc.nsteps, st, out=out, saved=saved)
+                cc.adjoint(0, cc.nsteps, saved, lam, gin, grads)
+
+            run()
+            torch.cuda.synchronize(dev)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(self.reps):
+                run()
+            e1.record()
+            torch.cuda.synchronize(dev)
+            times[K] = e0.elapsed_time(e1) * 1e3 / self.reps
+        best = min(times, key=times.get)
+        self.cache[key] = {"fuse": best, "us": {str(k): round(v, 2) for k, v in times.items()}}
+        self._save()
+        return best
+
+
+# --------------------------------------------------------------------- batches
+class BatchEvaluator:
+    """Gradient of the total mean photon number of many independent pure-state
+    circuits, replayed from ONE captured HIP graph.
+
+    Each circuit is planned with the autotuned stage tile and owns its buffers
+    (every step input, cotangents, gradients).  The capture forks the circuits
+    over `nstreams` streams (independent graph branches the GPU runs
+    concurrently, so small circuits fill the compute units left idle by large
+    ones) and joins them before the graph ends.  A call copies the stacked
+    parameters into the static buffers and launches the graph once.
+    """
+
+    def __init__(self, circuits, dtype=torch.complex64, device="cuda", nstreams=8, tuner=None, use_graph=True):
+        lib = _lib.load()
+        dev = torch.device(device)
+        self.device = dev
+        self.tuner = tuner if tuner is not None else Autotuner()
+        self.items = []
+        offset = 0
+        for c in circuits:
+            ops = [(o[0], tuple(o[1])) for o in c["ops"]]
+            n, D = c["n"], c["D"]
+            K = self.tuner.best(ops, n, D, False, dtype, dev)
+            cc = CompiledCircuit(ops, n, D, False, dtype, dev, fuse=K)
+            st = torch.zeros([D] * n, dtype=dtype, device=dev)
+            st.view(-1)[0] = 1.0
+            it = {
+                "cc": cc,
+                "n": n,
+                "D": D,
+                "off": offset,
+                "params": torch.zeros((len(ops), 2), dtype=torch.float64, device=dev),
+                "saved": [st] + [cc.new_state(i) for i in range(1, cc.nsteps)],
+                "psi": cc.new_state(cc.nsteps),
+                "g_in": cc.new_state(0),
+                "grads": torch.zeros(2 * len(ops), dtype=torch.float64, device=dev),
+                "part": torch.empty(max(1, lib.sfamd_reduce_partials(n, D, 0)), dtype=torch.float64, device=dev),
+            }
+            it["g_psi"] = torch.empty_like(it["psi"])
+            offset += len(ops)
+            self.items.append(it)
+        self.nparams_rows = offset
+        self.losses = torch.zeros(len(self.items), dtype=torch.float64, device=dev)
+        self.main = torch.cuda.Stream(device=dev)
+        self.streams = [torch.cuda.Stream(device=dev) for _ in range(nstreams)]
+        # largest circuits first, dealt round-robin so branches carry similar bytes
+        self.order = sorted(range(len(self.items)), key=lambda i: -self.items[i]["psi"].numel())
+        self.graph = None
+        torch.cuda.synchronize(dev)
+        self._enqueue_all()
+        torch.cuda.synchronize(dev)
+        if use_graph:
+            sp = ctypes.c_void_p(self.main.cuda_stream)
+            _lib.call("sfamd_graph_begin", sp)
+            try:
+                self._enqueue_all()
+            finally:
+                g = ctypes.c_void_p()
+                _lib.call("sfamd_graph_end", sp, ctypes.byref(g))
+            self.graph = g
+        torch.cuda.synchronize(dev)
+
+    def _enqueue_one(self, it, k):
+        cc = it["cc"]
+        cc.build_table(it["params"])
+        cc.forward(0, cc.nsteps, it["saved"][0], out=it["psi"], saved=it["saved"][: cc.nsteps])
+        _lib.call(
+            "sfamd_number_loss",
+            _lib.ptr(it["psi"]),
+            _lib.ptr(it["g_psi"]),
+            it["n"],
+            it["D"],
+            _lib.ptr(it["part"]),
+            _lib.ptr(self.losses[k : k + 1]),
+            cc.code,
+            _lib.stream_ptr(cc.device),
+        )
+        cc.adjoint(0, cc.nsteps, it["saved"][: cc.nsteps], it["g_psi"], it["g_in"], it["grads"])
+
+    def _enqueue_all(self):
+        fork = torch.cuda.Event()
+        fork.record(self.main)
+        joins = []
+        for si, s in enumerate(self.streams):
+            s.wait_event(fork)
+            with torch.cuda.stream(s):
+                for j in range(si, len(self.order), len(self.streams)):
+                    k = self.order[j]
+                    self._enqueue_one(self.items[k], k)
+            ev = torch.cuda.Event()
+            ev.record(s)
+            joins.append(ev)
+        for ev in joins:
+            self.main.wait_event(ev)
+
+    def __call__(self, params_list):
+        """params_list: one float64 [nops, 2] tensor per circuit. Returns
+        (losses [ncircuits], list of gradient tensors [nops, 2]) on the device."""
+        cur = torch.cuda.current_stream(self.device)
+        self.main.wait_stream(cur)
+        with torch.cuda.stream(self.main):
+            for it, p in zip(self.items, params_list):
+                it["params"].copy_(p.detach().reshape(it["params"].shape))
+            if self.graph is not None:
+                _lib.call("sfamd_graph_launch", self.graph, ctypes.c_void_p(self.main.cuda_stream))
+            else:
+                self._enqueue_all()
+        cur.wait_stream(self.main)
+        return self.losses, [it["grads"].reshape(-1, 2) for it in self.items]
+
+    def __del__(self):
+        g = getattr(self, "graph", None)
+        if g is not None:
+            try:
+                _lib.load().sfamd_graph_destroy(g)
+            except Exception:
+                pass
+            self.graph = None

@@ -514,3 +514,67 @@ def test_tf_path_comparator_gradient_matches_adjoint():
     assert (psi.detach() - psi2.detach()).abs().max() < 1e-12
     g1, g2 = p1.grad, p2.grad
     assert float((g1 - g2).norm() / g2.norm()) < 1e-10
+
+
+# ------------------------------------------------------------------ per-fiber kernels
+@pytest.mark.parametrize("dtype", [C128, C64])
+@pytest.mark.parametrize("pure,n,D", [(True, 4, 6), (True, 5, 4), (False, 3, 5), (False, 3, 4)])
+def test_per_fiber_kernels_match_oracle_and_fd(dtype, pure, n, D):
+    """States large enough that the outer axes take the per-fiber kernels
+    (innermost non-gate run >= 16): forward vs the oracle, gradient vs FD."""
+    rng = np.random.default_rng(200 + n * D)
+    ops = circuits.layered(n, 1, rng, sd=0.3, loss=not pure)
+    ref, rpure = fo.run_ops(ops, n, D, pure=pure)
+    be = run_backend(ops, n, D, dtype, pure=pure, fuse=0) if False else None
+    cc = runtime.CompiledCircuit([(o[0], o[1]) for o in ops], n, D, not pure, dtype, DEV, fuse=0)
+    rows = np.zeros((len(ops), 2))
+    for i, (_, _, p) in enumerate(ops):
+        rows[i, : len(p)] = p
+    params = torch.tensor(rows, dtype=torch.float64, device=DEV, requires_grad=True)
+    st0 = torch.as_tensor(fo.vacuum(n, D, pure=pure), dtype=dtype, device=DEV)
+    out = runtime.CircuitFunction.apply(params, st0, cc, None)
+    assert np.abs(to_np(out) - ref).max() < TOL[dtype] * 10
+    idx = np.indices([D] * n).sum(axis=0)
+    weights = (idx - 1.0) ** 2
+    (runtime.all_fock_probs(out, n, D, not rpure).to(torch.float64) * torch.as_tensor(weights, device=DEV)).sum().backward()
+    g = params.grad.cpu().numpy()
+    flat, g_fd = _fd_grad(ops, n, D, pure, None, weights)
+    gflat = np.array([g[i, j] for i, (_, _, p) in enumerate(ops) for j in range(len(p))])
+    rel = np.linalg.norm(gflat - g_fd) / np.linalg.norm(g_fd)
+    assert rel < (1e-8 if dtype == C128 else 1e-5), rel
+    del be, flat
+
+
+# ------------------------------------------------------------------ batched circuits
+def test_batch_evaluator_matches_per_circuit_autograd():
+    rng = np.random.default_rng(61)
+    circs = []
+    for n, D, L in [(2, 9, 2), (3, 6, 1), (4, 5, 2), (2, 20, 1), (5, 4, 1)]:
+        circs.append(dict(n=n, D=D, ops=circuits.layered(n, L, rng, sd=0.3)))
+    tuner = runtime.Autotuner(path="")
+    be = runtime.BatchEvaluator(circs, C64, DEV, nstreams=3, tuner=tuner)
+    plist = []
+    for c in circs:
+        rows = np.zeros((len(c["ops"]), 2))
+        for i, (_, _, p) in enumerate(c["ops"]):
+            rows[i, : len(p)] = p
+        plist.append(torch.tensor(rows, dtype=torch.float64, device=DEV))
+    for rep in range(3):
+        scaled = [p * (1 + 0.05 * rep) for p in plist]
+        losses, grads = be(scaled)
+        losses = losses.clone()
+        grads = [g.clone() for g in grads]
+        for k, c in enumerate(circs):
+            n, D = c["n"], c["D"]
+            cc = runtime.CompiledCircuit([(o[0], o[1]) for o in c["ops"]], n, D, False, C64, DEV, fuse=0)
+            p = scaled[k].clone().requires_grad_(True)
+            vac = torch.zeros([D] * n, dtype=C64, device=DEV)
+            vac.view(-1)[0] = 1
+            psi = runtime.CircuitFunction.apply(p, vac, cc, None)
+            probs = runtime.all_fock_probs(psi, n, D, False).to(torch.float64)
+            w = torch.as_tensor(np.indices([D] * n).sum(axis=0), dtype=torch.float64, device=DEV)
+            loss = (probs * w).sum()
+            loss.backward()
+            assert abs(float(loss) - float(losses[k])) < 1e-5 * max(1.0, abs(float(loss)))
+            rel = float((p.grad - grads[k]).norm() / max(p.grad.norm(), 1e-30))
+            assert rel < 1e-4, (k, rel)
