@@ -48,7 +48,7 @@ def parse():
     ap.add_argument(
         "--workload",
         default="c3",
-        choices=["c3", "c4", "c5", "hbm"],
+        choices=["c2", "c3", "c4", "c5", "hbm"],
         help="c3: 6-mode ket gradient (default line); c4: 256 uneven circuits in one graph launch; "
         "c5: 3-mode cutoff-20 density-matrix gradient with loss and layer checkpoints; "
         "hbm: per-kernel HBM rates on a 2 GB ket and the 512 MB DM",
@@ -181,6 +181,73 @@ def run_c5(args):
     print(json.dumps(line))
 
 
+def run_c2(args):
+    """4-mode pure ket at cutoff 15 (50,625 amplitudes), 8 layers (256 ops),
+    forward only in complex64 and complex128: evals/s, norm and fock_prob; the
+    CPU leg times the oracle's faithful fock-backend loop and also yields the
+    complex128 reference state the GPU results are compared with."""
+    import numpy as np
+    import torch
+
+    from strawberryfields_amd import circuits, runtime
+
+    dev = torch.device("cuda:0")
+    c2 = circuits.config2()
+    n, D, ops = c2["n"], c2["D"], c2["ops"]
+    rows = np.zeros((len(ops), 2))
+    for i, (_, _, p) in enumerate(ops):
+        rows[i, : len(p)] = p
+    params = torch.tensor(rows, dtype=torch.float64, device=dev)
+    tuner = runtime.Autotuner(path=os.environ.get("SFAMD_AUTOTUNE_CACHE"))
+    out = {}
+    kets = {}
+    for name, dtype in (("complex64", torch.complex64), ("complex128", torch.complex128)):
+        K = tuner.best([(o[0], o[1]) for o in ops], n, D, False, dtype, dev)
+        cc = runtime.CompiledCircuit([(o[0], o[1]) for o in ops], n, D, False, dtype, dev, fuse=K)
+        vac = torch.zeros([D] * n, dtype=dtype, device=dev)
+        vac.view(-1)[0] = 1.0
+        psi = runtime.run_circuit(cc, params, vac)
+        torch.cuda.synchronize()
+        reps = max(10, args.steps)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            psi = runtime.run_circuit(cc, params, vac)
+        torch.cuda.synchronize()
+        t = (time.perf_counter() - t0) / reps
+        probs = runtime.all_fock_probs(psi, n, D, False)
+        kets[name] = psi.cpu().numpy()
+        out[name] = {
+            "fwd_evals_per_s": round(1.0 / t, 1),
+            "ms": round(t * 1e3, 4),
+            "fused_local_modes": K,
+            "steps": cc.nsteps,
+            "norm": float(runtime.norm2(psi, n, D, False)),
+            "fock_prob_0000": float(probs[0, 0, 0, 0]),
+            "fock_prob_1021": float(probs[1, 0, 2, 1]),
+        }
+    cpu = None
+    if not args.no_cpu:
+        from threadpoolctl import threadpool_limits
+
+        from oracle import fock_oracle as fo
+
+        with threadpool_limits(limits=1):
+            t0 = time.perf_counter()
+            ref, _ = fo.run_ops(ops, n, D, pure=True, faithful=True)
+            t_cpu = time.perf_counter() - t0
+        cpu = {"value": 1.0 / t_cpu, "unit": "forward evals/s", "cores": 1, "kind": "port",
+               "sample": "one complex128 forward through the oracle's faithful fock-backend loop (%.2f s)" % t_cpu}
+        r = ref.reshape(-1)
+        for name in kets:
+            k = kets[name].reshape(-1).astype(np.complex128)
+            out[name]["max_abs_dpsi"] = float(np.abs(k - r).max())
+            out[name]["one_minus_fidelity"] = float(
+                1 - abs(np.vdot(r, k)) ** 2 / (np.vdot(r, r).real * np.vdot(k, k).real)
+            )
+    print(json.dumps({"metric": "forward evals/s, 4-mode cutoff-15 ket, 256 ops", "results": out, "cpu_baseline": cpu,
+                      "dtype": "complex64 and complex128", "data": "synthetic", "n_gpus": 1}))
+
+
 def run_c4(args):
     """256 independent pure circuits (n in 2..5, cutoff 6..20, D^n <= 2^20,
     2-6 layers): forward + adjoint gradient of each circuit's total mean photon
@@ -278,6 +345,8 @@ def main():
         return run_c5(args)
     if args.workload == "c4":
         return run_c4(args)
+    if args.workload == "c2":
+        return run_c2(args)
     if args.workload == "hbm":
         return run_hbm(args)
     c3 = circuits.config3()
