@@ -153,7 +153,9 @@ def run_c5(args):
         torch.cuda.synchronize()
         t_fwd = (time.perf_counter() - t0) / reps
     tr = float(runtime.norm2(rho, n, D, True))
-    herm = float((rho.reshape(D**n, D**n) - rho.reshape(D**n, D**n).conj().T).abs().max())
+    perm = [2 * k for k in range(n)] + [2 * k + 1 for k in range(n)]
+    mat = rho.permute(*perm).reshape(D**n, D**n)
+    herm = float((mat - mat.conj().T).abs().max())
     S, rates = kernel_rates(n, D, True, dtype)
     nops = len(ops)
     fwd_alg = 2 * S * nops

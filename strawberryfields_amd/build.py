@@ -49,15 +49,30 @@ def _compile(src, obj, verbose):
     return obj
 
 
-def build(verbose=False, jobs=8):
-    """Compile every HIP source and link libsfamd.so; returns the library path."""
+def _stale(src, obj, headers):
+    if not os.path.exists(obj):
+        return True
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(f) > t for f in [src] + headers)
+
+
+def build(verbose=False, jobs=8, changed_only=False):
+    """Compile every HIP source and link libsfamd.so; returns the library path.
+
+    changed_only: recompile only sources newer than their object (or than any
+    header); the default rebuilds everything."""
     os.makedirs(BUILD_DIR, exist_ok=True)
     srcs = sorted(f for f in os.listdir(CSRC) if f.endswith(".hip"))
     objs = [os.path.join(BUILD_DIR, f[:-4] + ".o") for f in srcs]
+    headers = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
+    headers.append(os.path.join(REPO_DIR, "include", "sfamd.h"))
+    todo = [
+        (os.path.join(CSRC, s), o)
+        for s, o in zip(srcs, objs)
+        if not changed_only or _stale(os.path.join(CSRC, s), o, headers)
+    ]
     with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
-        futs = [
-            ex.submit(_compile, os.path.join(CSRC, s), o, verbose) for s, o in zip(srcs, objs)
-        ]
+        futs = [ex.submit(_compile, s, o, verbose) for s, o in todo]
         for f in futs:
             f.result()
     tmp = LIB_PATH + ".tmp"
@@ -72,4 +87,4 @@ def build(verbose=False, jobs=8):
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose=True, changed_only="--changed" in sys.argv))

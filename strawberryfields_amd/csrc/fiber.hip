@@ -132,38 +132,43 @@ k_axis_dense_adj_fib(const C<T>* __restrict__ psi, const C<T>* __restrict__ lo_,
 }
 
 // ------------------------------------------------------------------ beamsplitter
-// work item w = s * NF + f: block s of fiber f
+// Work is split into tasks (block s, chunk of kBlock fibers) with s taken from
+// the task index, which is uniform over the workgroup: every coefficient
+// address is then wave-uniform and the block's matrix is read through scalar
+// loads.  Amplitudes of block s are indexed relative to lo = bs_lo(s):
+// t = N - lo, at offset t * (sp - sq) from the block's first amplitude.
 template <typename T, int D>
 __global__ void __launch_bounds__(256)
 k_pair_bs_fib(const C<T>* __restrict__ in, C<T>* __restrict__ out, int64_t NF, Geo2 g, int64_t sp, int64_t sq,
               const C<T>* __restrict__ blk, int conj) {
-  const int64_t NW = (int64_t)(2 * D - 1) * NF;
-  GRID_STRIDE(w, NW) {
-    const int s = (int)(w / NF);
-    const int64_t f = w - (int64_t)s * NF;
-    const int64_t base = fiber_base2(f, g, D);
-    const int lo = s - D + 1 > 0 ? s - D + 1 : 0;
-    const int hi = s < D - 1 ? s : D - 1;
-    const int c = hi - lo + 1;
+  constexpr int NSB = 2 * D - 1;
+  const int64_t ntask = (NF + kBlock - 1) / kBlock * NSB;
+  for (int64_t task = blockIdx.x; task < ntask; task += gridDim.x) {
+    const int s = (int)(task % NSB);
+    const int64_t f = (task / NSB) * kBlock + threadIdx.x;
+    if (f >= NF) continue;
+    const int lo = bs_lo(s, D);
+    const int c = bs_hi(s, D) - lo + 1;
     const C<T>* bb = blk + bs_block_off(s, D);
+    const int64_t base = fiber_base2(f, g, D) + lo * sp + (s - lo) * sq;
+    const int64_t step = sp - sq;
     C<T> x[D];
 #pragma unroll
-    for (int k = 0; k < D; ++k)
-      if (k >= lo && k <= hi) x[k] = in[base + k * sp + (s - k) * sq];
+    for (int t = 0; t < D; ++t)
+      if (t < c) x[t] = in[base + t * step];
 #pragma unroll
-    for (int N = 0; N < D; ++N) {
-      if (N < lo || N > hi) continue;
+    for (int u = 0; u < D; ++u) {
+      if (u >= c) continue;
       C<T> acc = mk<T>(0, 0);
 #pragma unroll
-      for (int k = 0; k < D; ++k) {
-        if (k < lo || k > hi) continue;
-        const C<T> m = bb[(N - lo) * c + (k - lo)];
+      for (int t = 0; t < D; ++t) {
+        if (t >= c) continue;
         if (conj)
-          cmac_conj(acc, m, x[k]);
+          cmac_conj(acc, bb[u * c + t], x[t]);
         else
-          cmac(acc, m, x[k]);
+          cmac(acc, bb[u * c + t], x[t]);
       }
-      out[base + N * sp + (s - N) * sq] = acc;
+      out[base + u * step] = acc;
     }
   }
 }
@@ -173,48 +178,51 @@ __global__ void __launch_bounds__(256)
 k_pair_bs_adj_fib(const C<T>* __restrict__ psi, const C<T>* __restrict__ lo_, C<T>* __restrict__ li, int64_t NF,
                   Geo2 g, int64_t sp, int64_t sq, const C<T>* __restrict__ blk, const C<T>* __restrict__ dblk,
                   int conj, double* partial, int nwg, int zp1) {
-  const int64_t NW = (int64_t)(2 * D - 1) * NF;
+  constexpr int NSB = 2 * D - 1;
   const T sg = conj ? (T)-1 : (T)1;
+  const int64_t ntask = (NF + kBlock - 1) / kBlock * NSB;
   double g0 = 0.0, g1 = 0.0;
-  GRID_STRIDE(w, NW) {
-    const int s = (int)(w / NF);
-    const int64_t f = w - (int64_t)s * NF;
-    const int64_t base = fiber_base2(f, g, D);
-    const int lo = s - D + 1 > 0 ? s - D + 1 : 0;
-    const int hi = s < D - 1 ? s : D - 1;
-    const int c = hi - lo + 1;
-    const int64_t off = bs_block_off(s, D);
+  for (int64_t task = blockIdx.x; task < ntask; task += gridDim.x) {
+    const int s = (int)(task % NSB);
+    const int64_t f = (task / NSB) * kBlock + threadIdx.x;
+    if (f >= NF) continue;
+    const int lo = bs_lo(s, D);
+    const int c = bs_hi(s, D) - lo + 1;
+    const C<T>* bb = blk + bs_block_off(s, D);
+    const C<T>* db = dblk + bs_block_off(s, D);
+    const int64_t base = fiber_base2(f, g, D) + lo * sp + (s - lo) * sq;
+    const int64_t step = sp - sq;
     C<T> x[D], l[D];
 #pragma unroll
-    for (int k = 0; k < D; ++k)
-      if (k >= lo && k <= hi) {
-        x[k] = psi[base + k * sp + (s - k) * sq];
-        l[k] = lo_[base + k * sp + (s - k) * sq];
+    for (int t = 0; t < D; ++t)
+      if (t < c) {
+        x[t] = psi[base + t * step];
+        l[t] = lo_[base + t * step];
       }
     T s0 = 0, s1 = 0;
 #pragma unroll
     for (int n = 0; n < D; ++n) {
-      if (n < lo || n > hi) continue;
+      if (n >= c) continue;
       // lam_in[n] = sum_N conj(B'[N][n]) lam[N];  w0 = sum_N conj(lam[N]) dB'[N][n];
-      // w1 = sum_N conj(lam[N]) i sg (n - N) B'[N][n]
+      // w1 = sum_N conj(lam[N]) i sg (n - N) B'[N][n]   (indices relative to lo)
       C<T> acc = mk<T>(0, 0), w0 = mk<T>(0, 0), w1 = mk<T>(0, 0);
 #pragma unroll
       for (int N = 0; N < D; ++N) {
-        if (N < lo || N > hi) continue;
-        C<T> b = blk[off + (N - lo) * c + (n - lo)];
-        C<T> db = dblk[off + (N - lo) * c + (n - lo)];
+        if (N >= c) continue;
+        C<T> b = bb[N * c + n];
+        C<T> d = db[N * c + n];
         if (conj) {
           b.im = -b.im;
-          db.im = -db.im;
+          d.im = -d.im;
         }
         cmac_conj(acc, b, l[N]);
-        cmac_conj(w0, l[N], db);
+        cmac_conj(w0, l[N], d);
         const C<T> lb = mk<T>(l[N].re * b.re + l[N].im * b.im, l[N].re * b.im - l[N].im * b.re);
         const T wt = sg * (T)(n - N);
         w1.re += -wt * lb.im;
         w1.im += wt * lb.re;
       }
-      li[base + n * sp + (s - n) * sq] = acc;
+      li[base + n * step] = acc;
       s0 += w0.re * x[n].re - w0.im * x[n].im;
       s1 += w1.re * x[n].re - w1.im * x[n].im;
     }
@@ -225,28 +233,32 @@ k_pair_bs_adj_fib(const C<T>* __restrict__ psi, const C<T>* __restrict__ lo_, C<
 }
 
 // ------------------------------------------------------------------ loss
-// (bra, ket) axes of one mode: strides sp = D * sq, sq = B.  Work item
-// w = (d + D - 1) * NF + f: diagonal q - p = d of fiber f = (a, b).
+// (bra, ket) axes of one mode: strides sp = D * sq, sq = B.  Tasks are
+// (diagonal d = q - p, chunk of kBlock fibers f = (a, b)); d is uniform over
+// the workgroup, so the coefficient reads are scalar loads.
 template <typename T, int D>
 __global__ void __launch_bounds__(256)
 k_loss_fib(const C<T>* __restrict__ in, C<T>* __restrict__ out, int64_t NF, int64_t B, const T* __restrict__ coef) {
-  const int64_t NW = (int64_t)(2 * D - 1) * NF;
-  GRID_STRIDE(w, NW) {
-    const int d = (int)(w / NF) - (D - 1);
-    const int64_t f = w % NF;
+  constexpr int NSB = 2 * D - 1;
+  const int64_t ntask = (NF + kBlock - 1) / kBlock * NSB;
+  for (int64_t task = blockIdx.x; task < ntask; task += gridDim.x) {
+    const int d = (int)(task % NSB) - (D - 1);
+    const int64_t f = (task / NSB) * kBlock + threadIdx.x;
+    if (f >= NF) continue;
     const int64_t a = f / B, b = f - a * B;
-    const int64_t base = a * D * D * B + b;
     const int p0 = d < 0 ? -d : 0;  // first p with q = p + d >= 0
     const int L = D - (d < 0 ? -d : d);
+    const int64_t base = a * D * D * B + b + (int64_t)(p0 * D + p0 + d) * B;
+    const int64_t step = (int64_t)(D + 1) * B;
+    const T* cf0 = coef + ((int64_t)p0 * D + (p0 + d)) * D;
     C<T> x[D];
 #pragma unroll
     for (int t = 0; t < D; ++t)
-      if (t < L) x[t] = in[base + ((p0 + t) * D + (p0 + t + d)) * B];
+      if (t < L) x[t] = in[base + t * step];
 #pragma unroll
     for (int t = 0; t < D; ++t) {
       if (t >= L) continue;
-      const int p = p0 + t, q = p + d;
-      const T* cf = coef + ((int64_t)p * D + q) * D;
+      const T* cf = cf0 + t * (D + 1) * D;
       C<T> acc = mk<T>(0, 0);
 #pragma unroll
       for (int l = 0; l < D; ++l) {
@@ -254,7 +266,7 @@ k_loss_fib(const C<T>* __restrict__ in, C<T>* __restrict__ out, int64_t NF, int6
         acc.re += cf[l] * x[t + l].re;
         acc.im += cf[l] * x[t + l].im;
       }
-      out[base + (p * D + q) * B] = acc;
+      out[base + t * step] = acc;
     }
   }
 }
@@ -264,40 +276,42 @@ __global__ void __launch_bounds__(256)
 k_loss_adj_fib(const C<T>* __restrict__ psi, const C<T>* __restrict__ lo_, C<T>* __restrict__ li, int64_t NF,
                int64_t B, const T* __restrict__ coef, const T* __restrict__ dcoef, double* partial, int nwg,
                int zp1) {
-  const int64_t NW = (int64_t)(2 * D - 1) * NF;
+  constexpr int NSB = 2 * D - 1;
+  const int64_t ntask = (NF + kBlock - 1) / kBlock * NSB;
   double g0 = 0.0;
-  GRID_STRIDE(w, NW) {
-    const int d = (int)(w / NF) - (D - 1);
-    const int64_t f = w % NF;
+  for (int64_t task = blockIdx.x; task < ntask; task += gridDim.x) {
+    const int d = (int)(task % NSB) - (D - 1);
+    const int64_t f = (task / NSB) * kBlock + threadIdx.x;
+    if (f >= NF) continue;
     const int64_t a = f / B, b = f - a * B;
-    const int64_t base = a * D * D * B + b;
     const int p0 = d < 0 ? -d : 0;
     const int L = D - (d < 0 ? -d : d);
+    const int64_t base = a * D * D * B + b + (int64_t)(p0 * D + p0 + d) * B;
+    const int64_t step = (int64_t)(D + 1) * B;
+    const int64_t c0 = ((int64_t)p0 * D + (p0 + d)) * D;  // coefficient row of position t = 0
     C<T> x[D], l[D];
 #pragma unroll
     for (int t = 0; t < D; ++t)
       if (t < L) {
-        const int64_t idx = base + ((p0 + t) * D + (p0 + t + d)) * B;
-        x[t] = psi[idx];
-        l[t] = lo_[idx];
+        x[t] = psi[base + t * step];
+        l[t] = lo_[base + t * step];
       }
     T s0 = 0;
 #pragma unroll
     for (int t = 0; t < D; ++t) {
       if (t >= L) continue;
       // lam_in at position t: sum over outputs u = t - lp (lp >= 0) of c[u][lp] lam[u]
-      const int p = p0 + t, q = p + d;
       C<T> acc = mk<T>(0, 0);
 #pragma unroll
       for (int lp = 0; lp < D; ++lp) {
         if (lp > t) continue;
         const int u = t - lp;
-        const T cc = coef[((int64_t)(p - lp) * D + (q - lp)) * D + lp];
+        const T cc = coef[c0 + (int64_t)u * (D + 1) * D + lp];
         acc.re += cc * l[u].re;
         acc.im += cc * l[u].im;
       }
-      li[base + (p * D + q) * B] = acc;
-      const T* dc = dcoef + ((int64_t)p * D + q) * D;
+      li[base + t * step] = acc;
+      const T* dc = dcoef + c0 + (int64_t)t * (D + 1) * D;
       C<T> u = mk<T>(0, 0);
 #pragma unroll
       for (int lp = 0; lp < D; ++lp) {
@@ -324,6 +338,13 @@ Geo2 geo2(int64_t sp, int64_t sq, int64_t N, int D) {
   g.C = g.s_lo / (D * g.s_hi);
   (void)N;
   return g;
+}
+
+// one workgroup per (block or diagonal, chunk of kBlock fibers), capped
+int task_grid(int64_t NF, int D) {
+  int64_t t = (NF + kBlock - 1) / kBlock * (2 * D - 1);
+  if (t > kMaxGrid) t = kMaxGrid;
+  return (int)t;
 }
 
 }  // namespace
@@ -385,7 +406,7 @@ int pair_bs_fib(const void* in, void* out, int64_t N, int D, int64_t sp, int64_t
                 int dtype, hipStream_t s) {
   const int64_t NF = N / ((int64_t)D * D);
   const Geo2 g = geo2(sp, sq, N, D);
-  const int grid = grid_for((2 * D - 1) * NF);
+  const int grid = task_grid(NF, D);
   switch (D) {
 #define CASE(d)                                                                                                    \
   case d:                                                                                                          \
@@ -431,7 +452,7 @@ int pair_bs_adj_fib(const void* psi, const void* lam_out, void* lam_in, int64_t 
 
 int loss_fib(const void* in, void* out, int64_t N, int D, int64_t B, const void* coef, int dtype, hipStream_t s) {
   const int64_t NF = N / ((int64_t)D * D);
-  const int grid = grid_for((2 * D - 1) * NF);
+  const int grid = task_grid(NF, D);
   switch (D) {
 #define CASE(d)                                                                                                  \
   case d:                                                                                                        \

@@ -1,0 +1,18 @@
+// Matrix-core kernels for dense single-mode gates (complex64).
+#pragma once
+
+#include "common.h"
+
+namespace sfamd {
+
+constexpr int kMfmaMaxD = 20;
+
+bool mfma_supported(int D, int dtype);
+// out = U' in along an axis of stride R (U' = U, or conj(U) if conj)
+int axis_dense_mfma(const void* in, void* out, int64_t N, int D, int64_t R, const void* M, int conj, hipStream_t s);
+// lam_in = U'^H lam_out and the two parameter partial sums (grid = nwg)
+int axis_dense_adj_mfma(const void* psi, const void* lam_out, void* lam_in, int64_t N, int D, int64_t R,
+                        const void* M, const void* dM0, const void* dM1, int conj, double* partial, int nwg, int zp1,
+                        hipStream_t s);
+
+}  // namespace sfamd

@@ -14,6 +14,8 @@
 #include "common.h"
 #include "fiber.h"
 #include "fused.h"
+#include "mfma.h"
+#include "slab.h"
 
 using namespace sfamd;
 
@@ -54,8 +56,10 @@ int validate_op(int kind, int m0, int m1, int n, int D) {
 // is compiled for D and the innermost non-gate run is long enough to coalesce.
 bool use_fib(int D, int64_t run) { return fiber_supported(D) && run >= kFiberMinRun; }
 
+// complex64 dense gates run on the matrix cores for every axis stride
 int dense_pass(const void* in, void* out, int64_t N, int D, int64_t B, const void* U, int conj, int parity, int dtype,
                hipStream_t s) {
+  if (mfma_supported(D, dtype)) return axis_dense_mfma(in, out, N, D, B, U, conj, s);
   if (use_fib(D, B)) return axis_dense_fib(in, out, N, D, B, U, conj, parity, dtype, s);
   return axis_dense(in, out, N, D, B, U, conj, dtype, s);
 }
@@ -63,31 +67,48 @@ int dense_pass(const void* in, void* out, int64_t N, int D, int64_t B, const voi
 int dense_adj_pass(const void* psi, const void* lo, void* li, int64_t N, int D, int64_t B, const void* U,
                    const void* dU0, const void* dU1, int conj, int parity, double* pb, int nwg, int zp1, int dtype,
                    hipStream_t s) {
+  if (mfma_supported(D, dtype)) return axis_dense_adj_mfma(psi, lo, li, N, D, B, U, dU0, dU1, conj, pb, nwg, zp1, s);
   if (use_fib(D, B)) return axis_dense_adj_fib(psi, lo, li, N, D, B, U, dU0, dU1, conj, parity, pb, nwg, zp1, dtype, s);
   return axis_dense_adj(psi, lo, li, N, D, B, U, dU0, dU1, conj, pb, nwg, zp1, dtype, s);
+}
+
+// Two-axis gates: per-fiber kernels when the inner gate axis has a long
+// coalesced run; LDS slab tiles when it is (nearly) innermost and a tile fits;
+// element-wise kernels otherwise.  Slab tiles need a large enough state to pay
+// for their staging, so tiny states stay element-wise.
+constexpr int64_t kSlabMinState = 1 << 16;
+
+bool use_slab(int64_t N, int D, int64_t sa, int64_t sb, int dtype, int adjoint) {
+  const int64_t so = sa > sb ? sa : sb, si = sa > sb ? sb : sa;
+  return N >= kSlabMinState && slab_supported(N, D, so, si, dtype, adjoint);
 }
 
 int bs_pass(const void* in, void* out, int64_t N, int D, int64_t sp, int64_t sq, const void* g, int conj, int dtype,
             hipStream_t s) {
   if (use_fib(D, sp < sq ? sp : sq)) return pair_bs_fib(in, out, N, D, sp, sq, g, conj, dtype, s);
+  if (use_slab(N, D, sp, sq, dtype, 0)) return pair_bs_slab(in, out, N, D, sp, sq, g, conj, dtype, s);
   return pair_bs(in, out, N, D, sp, sq, g, conj, dtype, s);
 }
 
 int bs_adj_pass(const void* psi, const void* lo, void* li, int64_t N, int D, int64_t sp, int64_t sq, const void* g,
                 const void* dB, int conj, double* pb, int nwg, int zp1, int dtype, hipStream_t s) {
   if (use_fib(D, sp < sq ? sp : sq)) return pair_bs_adj_fib(psi, lo, li, N, D, sp, sq, g, dB, conj, pb, nwg, zp1, dtype, s);
+  if (use_slab(N, D, sp, sq, dtype, 1))
+    return pair_bs_adj_slab(psi, lo, li, N, D, sp, sq, g, dB, conj, pb, nwg, zp1, dtype, s);
   return pair_bs_adj(psi, lo, li, N, D, sp, sq, g, dB, conj, pb, nwg, zp1, dtype, s);
 }
 
 int loss_pass(const void* in, void* out, int64_t N, int D, int64_t sb, int64_t sk, const void* c, int dtype,
               hipStream_t s) {
   if (use_fib(D, sk)) return loss_fib(in, out, N, D, sk, c, dtype, s);
+  if (use_slab(N, D, sb, sk, dtype, 0)) return loss_slab(in, out, N, D, sb, sk, c, dtype, s);
   return loss_fwd(in, out, N, D, sb, sk, c, dtype, s);
 }
 
 int loss_adj_pass(const void* psi, const void* lo, void* li, int64_t N, int D, int64_t sb, int64_t sk, const void* c,
                   const void* dc, double* pb, int nwg, int zp1, int dtype, hipStream_t s) {
   if (use_fib(D, sk)) return loss_adj_fib(psi, lo, li, N, D, sk, c, dc, pb, nwg, zp1, dtype, s);
+  if (use_slab(N, D, sb, sk, dtype, 1)) return loss_adj_slab(psi, lo, li, N, D, sb, sk, c, dc, pb, nwg, zp1, dtype, s);
   return loss_adj(psi, lo, li, N, D, sb, sk, c, dc, pb, nwg, zp1, dtype, s);
 }
 

@@ -578,3 +578,61 @@ def test_batch_evaluator_matches_per_circuit_autograd():
             assert abs(float(loss) - float(losses[k])) < 1e-5 * max(1.0, abs(float(loss)))
             rel = float((p.grad - grads[k]).norm() / max(p.grad.norm(), 1e-30))
             assert rel < 1e-4, (k, rel)
+
+
+# ------------------------------------------------------------------ LDS slab kernels
+def _fd_subset(ops, n, D, pure, weights, idx, h=1e-5):
+    flat = np.array([v for _, _, p in ops for v in p], dtype=float)
+    g = np.zeros(len(idx))
+    for j, i in enumerate(idx):
+        a, b = flat.copy(), flat.copy()
+        a[i] += h
+        b[i] -= h
+        g[j] = (_oracle_loss(ops, n, D, pure, a, None, weights) - _oracle_loss(ops, n, D, pure, b, None, weights)) / (2 * h)
+    return g
+
+
+@pytest.mark.parametrize("dtype", [C128, C64])
+@pytest.mark.parametrize(
+    "pure,n,D,pairs",
+    [
+        (True, 4, 16, [(2, 3), (3, 2), (1, 2)]),  # innermost pair: slab tiles of D^2
+        (False, 2, 16, [(0, 1), (1, 0)]),  # ket pair (256, 1): one slab per tile
+        (False, 3, 7, [(1, 2), (2, 1), (0, 1)]),  # bra pair (343, 7): Sin = 7, mid axes
+        (False, 2, 20, [(0, 1)]),  # ket pair slab 8000: split mid range in the adjoint
+    ],
+)
+def test_slab_kernels_match_oracle_and_fd(dtype, pure, n, D, pairs):
+    """States of >= 2^16 amplitudes whose gates touch the innermost axes take
+    the LDS slab kernels (beamsplitter, loss) and the matrix-core kernel
+    (complex64 dense gates): forward vs the oracle, gradient vs FD on a seeded
+    subset of the parameters."""
+    rng = np.random.default_rng(300 + n * D)
+    ops = []
+    for m in range(n):
+        ops.append(("S", (m,), [float(rng.uniform(0.05, 0.3)), float(rng.uniform(0, 2 * np.pi))]))
+        ops.append(("D", (m,), [float(rng.uniform(0.05, 0.3)), float(rng.uniform(0, 2 * np.pi))]))
+    for p, q in pairs:
+        ops.append(("BS", (p, q), [float(rng.uniform(0.2, 1.2)), float(rng.uniform(0, 2 * np.pi))]))
+    if not pure:
+        for m in range(n):
+            ops.append(("LOSS", (m,), [float(rng.uniform(0.6, 0.95))]))
+    ops.append(("BS", (pairs[0][0], pairs[0][1]), [float(rng.uniform(0.2, 1.2)), float(rng.uniform(0, 2 * np.pi))]))
+    ref, rpure = fo.run_ops(ops, n, D, pure=pure)
+    cc = runtime.CompiledCircuit([(o[0], o[1]) for o in ops], n, D, not pure, dtype, DEV, fuse=0)
+    rows = np.zeros((len(ops), 2))
+    for i, (_, _, p) in enumerate(ops):
+        rows[i, : len(p)] = p
+    params = torch.tensor(rows, dtype=torch.float64, device=DEV, requires_grad=True)
+    st0 = torch.as_tensor(fo.vacuum(n, D, pure=pure), dtype=dtype, device=DEV)
+    out = runtime.CircuitFunction.apply(params, st0, cc, None)
+    assert np.abs(to_np(out) - ref).max() < TOL[dtype] * 10
+    idx = np.indices([D] * n).sum(axis=0)
+    weights = (idx - 1.0) ** 2
+    (runtime.all_fock_probs(out, n, D, not rpure).to(torch.float64) * torch.as_tensor(weights, device=DEV)).sum().backward()
+    g = params.grad.cpu().numpy()
+    gflat = np.array([g[i, j] for i, (_, _, p) in enumerate(ops) for j in range(len(p))])
+    sel = np.sort(rng.choice(len(gflat), size=min(8, len(gflat)), replace=False))
+    g_fd = _fd_subset(ops, n, D, pure, weights, sel)
+    rel = np.linalg.norm(gflat[sel] - g_fd) / np.linalg.norm(g_fd)
+    assert rel < (1e-7 if dtype == C128 else 1e-4), rel
