@@ -15,4 +15,12 @@ int axis_dense_adj_mfma(const void* psi, const void* lam_out, void* lam_in, int6
                         const void* M, const void* dM0, const void* dM1, int conj, double* partial, int nwg, int zp1,
                         hipStream_t s);
 
+// beamsplitter on axes of strides sp (first mode) and sq, blocks in the compact
+// layout of the gate table; use when the inner gate axis run is >= 16
+int pair_bs_mfma(const void* in, void* out, int64_t N, int D, int64_t sp, int64_t sq, const void* blocks, int conj,
+                 hipStream_t s);
+int pair_bs_adj_mfma(const void* psi, const void* lam_out, void* lam_in, int64_t N, int D, int64_t sp, int64_t sq,
+                     const void* blocks, const void* dblocks, int conj, double* partial, int nwg, int zp1,
+                     hipStream_t s);
+
 }  // namespace sfamd

@@ -31,56 +31,111 @@ namespace {
 typedef float f4 __attribute__((ext_vector_type(4)));
 using C = cpx<float>;
 
+// Register plan for an operator of up to 8 MB complex rows and columns.
+template <int MB>
+struct Blk {
+  static constexpr int Q = 2 * MB;   // amplitudes per lane group
+  static constexpr int NS = 4 * MB;  // k-steps (4 real columns each)
+};
+
 template <int D>
 struct Plan {
   static constexpr int Mb = (2 * D + 15) / 16;  // 16-row blocks
-  static constexpr int Q = 2 * Mb;              // amplitudes per lane group
-  static constexpr int NS = 4 * Mb;             // k-steps (4 real columns each)
+  static constexpr int Q = Blk<Mb>::Q;
+  static constexpr int NS = Blk<Mb>::NS;
 };
 
-// real-form entry of op(M) at row (i, a), column (k, b); a, b: 0 = re, 1 = im.
-// op: 0 -> M, 1 -> conj(M), 2 -> M^H, 3 -> M^T
-__device__ __forceinline__ float real_entry(const C* __restrict__ M, int D, int op, int i, int a, int k, int b) {
-  if (i >= D || k >= D) return 0.f;
-  const C m = op >= 2 ? M[k * D + i] : M[i * D + k];
-  const float im = (op == 1 || op == 2) ? -m.im : m.im;
-  if (a == b) return m.re;
-  return a ? im : -im;
-}
-
-// this lane's A operands of op(M): A[mb][s]
-template <int D>
-__device__ __forceinline__ void load_operator(const C* __restrict__ M, int op, float (&A)[Plan<D>::Mb][Plan<D>::NS]) {
-  constexpr int Q = Plan<D>::Q;
+// this lane's A operands of the real form of a complex operator given by
+// entry(i, k) (row i = output amplitude, column k = input amplitude; zero
+// outside the operator): A[mb][s]
+template <int MB, typename F>
+__device__ __forceinline__ void load_op(F entry, float (&A)[MB][Blk<MB>::NS]) {
+  constexpr int Q = Blk<MB>::Q;
   const int lane = threadIdx.x & 63;
   const int c = lane & 15, g = lane >> 4;
 #pragma unroll
-  for (int mb = 0; mb < Plan<D>::Mb; ++mb)
+  for (int mb = 0; mb < MB; ++mb)
 #pragma unroll
-    for (int s = 0; s < Plan<D>::NS; ++s)
-      A[mb][s] = real_entry(M, D, op, (c >> 2) * Q + 2 * mb + ((c & 3) >> 1), c & 1, g * Q + (s >> 1), s & 1);
+    for (int s = 0; s < Blk<MB>::NS; ++s) {
+      const int a = c & 1, b = s & 1;
+      const C m = entry((c >> 2) * Q + 2 * mb + ((c & 3) >> 1), g * Q + (s >> 1));
+      A[mb][s] = a == b ? m.re : (a ? m.im : -m.im);
+    }
 }
 
-// acc = op(M) x for this lane's 16-fiber block (x in group layout)
+// entry of op(M) for an n x n row-major matrix; op: 0 -> M, 1 -> conj(M),
+// 2 -> M^H, 3 -> M^T
+__device__ __forceinline__ C op_entry(const C* __restrict__ M, int n, int op, int i, int k) {
+  if (i >= n || k >= n) return mk<float>(0.f, 0.f);
+  C m = op >= 2 ? M[k * n + i] : M[i * n + k];
+  if (op == 1 || op == 2) m.im = -m.im;
+  return m;
+}
+
 template <int D>
-__device__ __forceinline__ void apply_operator(const float (&A)[Plan<D>::Mb][Plan<D>::NS], const C (&x)[Plan<D>::Q],
-                                               f4 (&acc)[Plan<D>::Mb]) {
+__device__ __forceinline__ void load_operator(const C* __restrict__ M, int op, float (&A)[Plan<D>::Mb][Plan<D>::NS]) {
+  load_op<Plan<D>::Mb>([&](int i, int k) { return op_entry(M, D, op, i, k); }, A);
+}
+
+// acc = op x for this lane's 16-fiber block (x in group layout)
+template <int MB>
+__device__ __forceinline__ void apply_op(const float (&A)[MB][Blk<MB>::NS], const C (&x)[Blk<MB>::Q], f4 (&acc)[MB]) {
 #pragma unroll
-  for (int mb = 0; mb < Plan<D>::Mb; ++mb) acc[mb] = f4{0.f, 0.f, 0.f, 0.f};
+  for (int mb = 0; mb < MB; ++mb) acc[mb] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int s = 0; s < Plan<D>::NS; ++s) {
+  for (int s = 0; s < Blk<MB>::NS; ++s) {
     const float b = (s & 1) ? x[s >> 1].im : x[s >> 1].re;
 #pragma unroll
-    for (int mb = 0; mb < Plan<D>::Mb; ++mb)
-      acc[mb] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[mb][s], b, acc[mb], 0, 0, 0);
+    for (int mb = 0; mb < MB; ++mb) acc[mb] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[mb][s], b, acc[mb], 0, 0, 0);
   }
 }
 
-// amplitude q of this lane's group from the accumulators
+// Operators shared by every wave of a workgroup (adjoint kernels) live in LDS
+// in lane order, dst[(mb * NS + s) * 64 + lane]; the block's threads build them
+// together and the A operand of each MFMA is one conflict-free ds_read.
+template <int MB, typename F>
+__device__ __forceinline__ void build_op_lds(F entry, float* dst) {
+  constexpr int Q = Blk<MB>::Q, NS = Blk<MB>::NS;
+  for (int e = threadIdx.x; e < MB * NS * 64; e += blockDim.x) {
+    const int lane = e & 63, slot = e >> 6;
+    const int mb = slot / NS, s = slot - mb * NS;
+    const int c = lane & 15, g = lane >> 4;
+    const int a = c & 1, b = s & 1;
+    const C m = entry((c >> 2) * Q + 2 * mb + ((c & 3) >> 1), g * Q + (s >> 1));
+    dst[e] = a == b ? m.re : (a ? m.im : -m.im);
+  }
+}
+
+template <int MB>
+__device__ __forceinline__ void apply_op_lds(const float* opl, const C (&x)[Blk<MB>::Q], f4 (&acc)[MB]) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int mb = 0; mb < MB; ++mb) acc[mb] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int s = 0; s < Blk<MB>::NS; ++s) {
+    const float b = (s & 1) ? x[s >> 1].im : x[s >> 1].re;
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+      acc[mb] = __builtin_amdgcn_mfma_f32_16x16x4f32(opl[(mb * Blk<MB>::NS + s) * 64 + lane], b, acc[mb], 0, 0, 0);
+  }
+}
+
 template <int D>
-__device__ __forceinline__ C acc_amp(const f4 (&acc)[Plan<D>::Mb], int q) {
+__device__ __forceinline__ void apply_operator(const float (&A)[Plan<D>::Mb][Plan<D>::NS], const C (&x)[Plan<D>::Q],
+                                               f4 (&acc)[Plan<D>::Mb]) {
+  apply_op<Plan<D>::Mb>(A, x, acc);
+}
+
+// amplitude q of this lane's group from the accumulators
+template <int MB>
+__device__ __forceinline__ C acc_at(const f4 (&acc)[MB], int q) {
   const f4 v = acc[q >> 1];
   return (q & 1) ? mk<float>(v[2], v[3]) : mk<float>(v[0], v[1]);
+}
+
+template <int D>
+__device__ __forceinline__ C acc_amp(const f4 (&acc)[Plan<D>::Mb], int q) {
+  return acc_at<Plan<D>::Mb>(acc, q);
 }
 
 // fiber f of an axis with stride R (R = 1: innermost) -> offset of amplitude 0
@@ -123,17 +178,21 @@ __global__ void __launch_bounds__(256)
 k_axis_mfma_adj(const C* __restrict__ psi, const C* __restrict__ lo, C* __restrict__ li, int64_t NF, int64_t R,
                 const C* __restrict__ M, const C* __restrict__ dM0, const C* __restrict__ dM1, int conj,
                 double* __restrict__ partial, int nwg, int zp1) {
-  constexpr int Q = Plan<D>::Q;
-  float AH[Plan<D>::Mb][Plan<D>::NS], A0[Plan<D>::Mb][Plan<D>::NS], A1[Plan<D>::Mb][Plan<D>::NS];
-  load_operator<D>(M, conj ? 3 : 2, AH);
-  load_operator<D>(dM0, conj ? 1 : 0, A0);
-  load_operator<D>(dM1, conj ? 1 : 0, A1);
+  constexpr int MB = Plan<D>::Mb, Q = Plan<D>::Q, OPN = MB * Plan<D>::NS * 64;
+  __shared__ float opl[3 * OPN];
+  build_op_lds<MB>([&](int i, int k) { return op_entry(M, D, conj ? 3 : 2, i, k); }, opl);
+  build_op_lds<MB>([&](int i, int k) { return op_entry(dM0, D, conj ? 1 : 0, i, k); }, opl + OPN);
+  build_op_lds<MB>([&](int i, int k) { return op_entry(dM1, D, conj ? 1 : 0, i, k); }, opl + 2 * OPN);
+  __syncthreads();
   const int lane = threadIdx.x & 63;
   const int c = lane & 15, g = lane >> 4;
   const int64_t nblk = (NF + 15) / 16;
   const int64_t nwave = (int64_t)gridDim.x * (blockDim.x >> 6);
   double g0 = 0.0, g1 = 0.0;
   for (int64_t blk = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); blk < nblk; blk += nwave) {
+    // keep the operators in LDS: without this barrier the compiler hoists
+    // their loop-invariant reads into (three operators' worth of) registers
+    __asm__ volatile("" ::: "memory");
     const int64_t f = blk * 16 + c;
     const bool valid = f < NF;
     const int64_t base = fiber_base(valid ? f : 0, R, D) + (int64_t)(g * Q) * R;
@@ -144,20 +203,20 @@ k_axis_mfma_adj(const C* __restrict__ psi, const C* __restrict__ lo, C* __restri
       x[q] = in_range ? psi[base + q * R] : mk<float>(0.f, 0.f);
       l[q] = in_range ? lo[base + q * R] : mk<float>(0.f, 0.f);
     }
-    f4 acc[Plan<D>::Mb];
-    apply_operator<D>(AH, l, acc);
+    f4 acc[MB];
+    apply_op_lds<MB>(opl, l, acc);
     if (valid) {
 #pragma unroll
       for (int q = 0; q < Q; ++q)
-        if (g * Q + q < D) li[base + q * R] = acc_amp<D>(acc, q);
+        if (g * Q + q < D) li[base + q * R] = acc_at<MB>(acc, q);
     }
     float s0 = 0.f, s1 = 0.f;
-    apply_operator<D>(A0, x, acc);
+    apply_op_lds<MB>(opl + OPN, x, acc);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) s0 += re_conj_mul(l[q], acc_amp<D>(acc, q));
-    apply_operator<D>(A1, x, acc);
+    for (int q = 0; q < Q; ++q) s0 += re_conj_mul(l[q], acc_at<MB>(acc, q));
+    apply_op_lds<MB>(opl + 2 * OPN, x, acc);
 #pragma unroll
-    for (int q = 0; q < Q; ++q) s1 += re_conj_mul(l[q], acc_amp<D>(acc, q));
+    for (int q = 0; q < Q; ++q) s1 += re_conj_mul(l[q], acc_at<MB>(acc, q));
     g0 += (double)s0;
     g1 += (double)s1;
   }
@@ -172,6 +231,183 @@ k_axis_mfma_adj(const C* __restrict__ psi, const C* __restrict__ lo, C* __restri
       partial[3 * nwg + blockIdx.x] = 0.0;
     }
   }
+}
+
+// ------------------------------------------------------------------ beamsplitter
+// The photon-number block s of a beamsplitter (c = c(s) <= D amplitudes,
+// fockbackend/circuit.py:306-335) is a c x c complex operator applied to the
+// fibers (N, s - N) of the two gate axes.  Work is split into tasks (s, chunk
+// of kBsChunk fibers): s is uniform over the workgroup, each wave builds the
+// block's real operator once per task (Mb = ceil(2c / 16) in 1..3) and applies
+// it to 16 fibers per MFMA chain.  Fibers of a chunk are consecutive inner
+// positions of the non-gate axes (coalesced when that run is >= 16).
+constexpr int kBsChunk = 1024;
+
+struct PairGeo {
+  int64_t B, C;        // extents after the inner gate axis, between the two gate axes
+  int64_t s_lo, s_hi;  // strides of the outer and inner gate axes
+};
+
+__device__ __forceinline__ int64_t pair_base(int64_t f, const PairGeo& g, int D) {
+  const int64_t b = f % g.B;
+  const int64_t t = f / g.B;
+  const int64_t c = t % g.C;
+  const int64_t a = t / g.C;
+  return a * (D * g.s_lo) + c * (D * g.s_hi) + b;
+}
+
+__device__ __forceinline__ void mfma_write_partials(double g0, double g1, double* partial, int nwg, int zp1) {
+  __shared__ double sh[8];
+  const double t0 = block_sum(g0, sh);
+  const double t1 = block_sum(g1, sh);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x] = t0;
+    partial[nwg + blockIdx.x] = t1;
+    if (zp1) {
+      partial[2 * nwg + blockIdx.x] = 0.0;
+      partial[3 * nwg + blockIdx.x] = 0.0;
+    }
+  }
+}
+
+// one task of a beamsplitter pass: block s (c amplitudes from N = lo) on the
+// fibers f0 .. f0 + kBsChunk - 1
+template <int MB, int D>
+__device__ __forceinline__ void bs_task(const C* __restrict__ in, C* __restrict__ out, int64_t NF, const PairGeo& geo,
+                                        int64_t sp, int64_t sq, const C* __restrict__ bb, int c, int lo, int s,
+                                        int conj, int64_t f0) {
+  constexpr int Q = Blk<MB>::Q;
+  float A[MB][Blk<MB>::NS];
+  load_op<MB>([&](int i, int k) { return op_entry(bb, c, conj ? 1 : 0, i, k); }, A);
+  const int lane = threadIdx.x & 63;
+  const int cl = lane & 15, g = lane >> 4;
+  const int64_t step = sp - sq;
+  const int64_t left = NF - f0;
+  const int nf = left < kBsChunk ? (int)left : kBsChunk;
+  for (int b16 = threadIdx.x >> 6; b16 * 16 < nf; b16 += blockDim.x >> 6) {
+    const bool valid = b16 * 16 + cl < nf;
+    const int64_t f = f0 + (valid ? b16 * 16 + cl : 0);
+    const int64_t base = pair_base(f, geo, D) + lo * sp + (s - lo) * sq + (int64_t)(g * Q) * step;
+    C x[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) x[q] = (valid && g * Q + q < c) ? in[base + q * step] : mk<float>(0.f, 0.f);
+    f4 acc[MB];
+    apply_op<MB>(A, x, acc);
+    if (valid) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q)
+        if (g * Q + q < c) out[base + q * step] = acc_at<MB>(acc, q);
+    }
+  }
+}
+
+template <int D>
+__global__ void __launch_bounds__(256)
+k_bs_mfma(const C* __restrict__ in, C* __restrict__ out, int64_t NF, PairGeo geo, int64_t sp, int64_t sq,
+          const C* __restrict__ blk, int conj) {
+  constexpr int NSB = 2 * D - 1;
+  const int64_t ntask = (NF + kBsChunk - 1) / kBsChunk * NSB;
+  for (int64_t task = blockIdx.x; task < ntask; task += gridDim.x) {
+    const int s = (int)(task % NSB);
+    const int64_t f0 = task / NSB * kBsChunk;
+    const int lo = bs_lo(s, D);
+    const int c = bs_hi(s, D) - lo + 1;
+    const C* bb = blk + bs_block_off(s, D);
+    const int mb = (2 * c + 15) / 16;
+    if (mb == 1)
+      bs_task<1, D>(in, out, NF, geo, sp, sq, bb, c, lo, s, conj, f0);
+    else if constexpr (D > 8) {
+      if (mb == 2)
+        bs_task<2, D>(in, out, NF, geo, sp, sq, bb, c, lo, s, conj, f0);
+      else if constexpr (D > 16)
+        bs_task<3, D>(in, out, NF, geo, sp, sq, bb, c, lo, s, conj, f0);
+    }
+  }
+}
+
+// adjoint of one task: lam_in = B'^H lam_out; gradients through dB'/dtheta
+// (table) and dB'/dphi = i sg (n - N) B'[N][n] (sg = -1 on the conjugated
+// side).  opl: the workgroup's LDS for three operators of MB blocks.
+template <int MB, int D>
+__device__ __forceinline__ void bs_adj_task(const C* __restrict__ psi, const C* __restrict__ lo_, C* __restrict__ li,
+                                            int64_t NF, const PairGeo& geo, int64_t sp, int64_t sq,
+                                            const C* __restrict__ bb, const C* __restrict__ db, int c, int lo, int s,
+                                            int conj, int64_t f0, float* opl, double& g0, double& g1) {
+  constexpr int Q = Blk<MB>::Q, OPN = MB * Blk<MB>::NS * 64;
+  __syncthreads();  // the previous task's operators are no longer read
+  build_op_lds<MB>([&](int i, int k) { return op_entry(bb, c, conj ? 3 : 2, i, k); }, opl);
+  build_op_lds<MB>([&](int i, int k) { return op_entry(db, c, conj ? 1 : 0, i, k); }, opl + OPN);
+  const float sg = conj ? -1.f : 1.f;
+  build_op_lds<MB>(
+      [&](int i, int k) {
+        const C m = op_entry(bb, c, conj ? 1 : 0, i, k);
+        const float w = sg * (float)(k - i);
+        return mk<float>(-w * m.im, w * m.re);
+      },
+      opl + 2 * OPN);
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int cl = lane & 15, g = lane >> 4;
+  const int64_t step = sp - sq;
+  const int64_t left = NF - f0;
+  const int nf = left < kBsChunk ? (int)left : kBsChunk;
+  for (int b16 = threadIdx.x >> 6; b16 * 16 < nf; b16 += blockDim.x >> 6) {
+    __asm__ volatile("" ::: "memory");  // operators stay in LDS (see k_axis_mfma_adj)
+    const bool valid = b16 * 16 + cl < nf;
+    const int64_t f = f0 + (valid ? b16 * 16 + cl : 0);
+    const int64_t base = pair_base(f, geo, D) + lo * sp + (s - lo) * sq + (int64_t)(g * Q) * step;
+    C x[Q], l[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const bool in_range = valid && g * Q + q < c;
+      x[q] = in_range ? psi[base + q * step] : mk<float>(0.f, 0.f);
+      l[q] = in_range ? lo_[base + q * step] : mk<float>(0.f, 0.f);
+    }
+    f4 acc[MB];
+    apply_op_lds<MB>(opl, l, acc);
+    if (valid) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q)
+        if (g * Q + q < c) li[base + q * step] = acc_at<MB>(acc, q);
+    }
+    float s0 = 0.f, s1 = 0.f;
+    apply_op_lds<MB>(opl + OPN, x, acc);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) s0 += re_conj_mul(l[q], acc_at<MB>(acc, q));
+    apply_op_lds<MB>(opl + 2 * OPN, x, acc);
+#pragma unroll
+    for (int q = 0; q < Q; ++q) s1 += re_conj_mul(l[q], acc_at<MB>(acc, q));
+    g0 += (double)s0;
+    g1 += (double)s1;
+  }
+}
+
+template <int D>
+__global__ void __launch_bounds__(256)
+k_bs_mfma_adj(const C* __restrict__ psi, const C* __restrict__ lo_, C* __restrict__ li, int64_t NF, PairGeo geo,
+              int64_t sp, int64_t sq, const C* __restrict__ blk, const C* __restrict__ dblk, int conj,
+              double* __restrict__ partial, int nwg, int zp1) {
+  constexpr int NSB = 2 * D - 1, MBX = Plan<D>::Mb;
+  __shared__ float opl[3 * MBX * Blk<MBX>::NS * 64];
+  const int64_t ntask = (NF + kBsChunk - 1) / kBsChunk * NSB;
+  double g0 = 0.0, g1 = 0.0;
+  for (int64_t task = blockIdx.x; task < ntask; task += gridDim.x) {
+    const int s = (int)(task % NSB);
+    const int64_t f0 = task / NSB * kBsChunk;
+    const int lo = bs_lo(s, D);
+    const int c = bs_hi(s, D) - lo + 1;
+    const int64_t off = bs_block_off(s, D);
+    const int mb = (2 * c + 15) / 16;
+    if (mb == 1)
+      bs_adj_task<1, D>(psi, lo_, li, NF, geo, sp, sq, blk + off, dblk + off, c, lo, s, conj, f0, opl, g0, g1);
+    else if constexpr (D > 8) {
+      if (mb == 2)
+        bs_adj_task<2, D>(psi, lo_, li, NF, geo, sp, sq, blk + off, dblk + off, c, lo, s, conj, f0, opl, g0, g1);
+      else if constexpr (D > 16)
+        bs_adj_task<3, D>(psi, lo_, li, NF, geo, sp, sq, blk + off, dblk + off, c, lo, s, conj, f0, opl, g0, g1);
+    }
+  }
+  mfma_write_partials(g0, g1, partial, nwg, zp1);
 }
 
 #define SFAMD_MFMA_D(X) \
@@ -209,6 +445,54 @@ int axis_dense_adj_mfma(const void* psi, const void* lam_out, void* lam_in, int6
   case d:                                                                                                          \
     hipLaunchKernelGGL((k_axis_mfma_adj<d>), dim3(nwg), dim3(kBlock), 0, s, (const C*)psi, (const C*)lam_out,     \
                        (C*)lam_in, NF, R, (const C*)M, (const C*)dM0, (const C*)dM1, conj, partial, nwg, zp1);    \
+    break;
+    SFAMD_MFMA_D(CASE)
+#undef CASE
+    default:
+      return SFAMD_E_CUTOFF;
+  }
+  return (int)hipGetLastError();
+}
+
+static PairGeo pair_geo(int64_t sp, int64_t sq, int D) {
+  PairGeo g;
+  g.s_lo = sp > sq ? sp : sq;
+  g.s_hi = sp > sq ? sq : sp;
+  g.B = g.s_hi;
+  g.C = g.s_lo / (D * g.s_hi);
+  return g;
+}
+
+int pair_bs_mfma(const void* in, void* out, int64_t N, int D, int64_t sp, int64_t sq, const void* blocks, int conj,
+                 hipStream_t s) {
+  const int64_t NF = N / ((int64_t)D * D);
+  const PairGeo g = pair_geo(sp, sq, D);
+  int64_t ntask = (NF + kBsChunk - 1) / kBsChunk * (2 * D - 1);
+  const int grid = (int)(ntask < kMaxGrid ? ntask : kMaxGrid);
+  switch (D) {
+#define CASE(d)                                                                                                 \
+  case d:                                                                                                       \
+    hipLaunchKernelGGL((k_bs_mfma<d>), dim3(grid), dim3(kBlock), 0, s, (const C*)in, (C*)out, NF, g, sp, sq,   \
+                       (const C*)blocks, conj);                                                                 \
+    break;
+    SFAMD_MFMA_D(CASE)
+#undef CASE
+    default:
+      return SFAMD_E_CUTOFF;
+  }
+  return (int)hipGetLastError();
+}
+
+int pair_bs_adj_mfma(const void* psi, const void* lam_out, void* lam_in, int64_t N, int D, int64_t sp, int64_t sq,
+                     const void* blocks, const void* dblocks, int conj, double* partial, int nwg, int zp1,
+                     hipStream_t s) {
+  const int64_t NF = N / ((int64_t)D * D);
+  const PairGeo g = pair_geo(sp, sq, D);
+  switch (D) {
+#define CASE(d)                                                                                                     \
+  case d:                                                                                                           \
+    hipLaunchKernelGGL((k_bs_mfma_adj<d>), dim3(nwg), dim3(kBlock), 0, s, (const C*)psi, (const C*)lam_out,        \
+                       (C*)lam_in, NF, g, sp, sq, (const C*)blocks, (const C*)dblocks, conj, partial, nwg, zp1);    \
     break;
     SFAMD_MFMA_D(CASE)
 #undef CASE

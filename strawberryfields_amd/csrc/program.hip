@@ -85,14 +85,20 @@ bool use_slab(int64_t N, int D, int64_t sa, int64_t sb, int dtype, int adjoint) 
 
 int bs_pass(const void* in, void* out, int64_t N, int D, int64_t sp, int64_t sq, const void* g, int conj, int dtype,
             hipStream_t s) {
-  if (use_fib(D, sp < sq ? sp : sq)) return pair_bs_fib(in, out, N, D, sp, sq, g, conj, dtype, s);
+  if (use_fib(D, sp < sq ? sp : sq)) {
+    if (mfma_supported(D, dtype)) return pair_bs_mfma(in, out, N, D, sp, sq, g, conj, s);
+    return pair_bs_fib(in, out, N, D, sp, sq, g, conj, dtype, s);
+  }
   if (use_slab(N, D, sp, sq, dtype, 0)) return pair_bs_slab(in, out, N, D, sp, sq, g, conj, dtype, s);
   return pair_bs(in, out, N, D, sp, sq, g, conj, dtype, s);
 }
 
 int bs_adj_pass(const void* psi, const void* lo, void* li, int64_t N, int D, int64_t sp, int64_t sq, const void* g,
                 const void* dB, int conj, double* pb, int nwg, int zp1, int dtype, hipStream_t s) {
-  if (use_fib(D, sp < sq ? sp : sq)) return pair_bs_adj_fib(psi, lo, li, N, D, sp, sq, g, dB, conj, pb, nwg, zp1, dtype, s);
+  if (use_fib(D, sp < sq ? sp : sq)) {
+    if (mfma_supported(D, dtype)) return pair_bs_adj_mfma(psi, lo, li, N, D, sp, sq, g, dB, conj, pb, nwg, zp1, s);
+    return pair_bs_adj_fib(psi, lo, li, N, D, sp, sq, g, dB, conj, pb, nwg, zp1, dtype, s);
+  }
   if (use_slab(N, D, sp, sq, dtype, 1))
     return pair_bs_adj_slab(psi, lo, li, N, D, sp, sq, g, dB, conj, pb, nwg, zp1, dtype, s);
   return pair_bs_adj(psi, lo, li, N, D, sp, sq, g, dB, conj, pb, nwg, zp1, dtype, s);
