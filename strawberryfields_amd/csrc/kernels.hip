@@ -55,6 +55,55 @@ k_axis_diag(const C<T>* __restrict__ in, C<T>* __restrict__ out, int64_t N, int 
   }
 }
 
+// ---- complex64 diagonal gates, 4 amplitudes per lane and step through
+// 16-byte nontemporal accesses (states are streamed once per gate).  The
+// axis positions i (stride Bi) and, for the two-sided DM form, j (stride Bj)
+// are found once per step and advanced by one amplitude at a time.
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+struct AxisPos {
+  int i;
+  int64_t r;  // offset within the current stride
+};
+
+__device__ __forceinline__ AxisPos axis_pos(int64_t e, int D, int64_t B) {
+  const int64_t q = e / B;
+  return AxisPos{(int)(q % D), e - q * B};
+}
+
+__device__ __forceinline__ void axis_next(AxisPos& p, int D, int64_t B) {
+  if (++p.r == B) {
+    p.r = 0;
+    if (++p.i == D) p.i = 0;
+  }
+}
+
+// factor of amplitude (i, j): d_i (conj: conj(d_i)) on one axis, d_i conj(d_j) on a DM mode
+__device__ __forceinline__ C<float> diag_factor(const C<float>* __restrict__ d, int i, int j, int pair, int conj) {
+  C<float> f = d[i];
+  if (pair) return cmul(f, cconj(d[j]));
+  if (conj) f.im = -f.im;
+  return f;
+}
+
+__global__ void __launch_bounds__(256)
+k_diag_c64(const f4v* __restrict__ in, f4v* __restrict__ out, int64_t N4, int D, int64_t Bi, int64_t Bj,
+           const C<float>* __restrict__ d, int pair, int conj) {
+  GRID_STRIDE(t, N4) {
+    AxisPos pi = axis_pos(4 * t, D, Bi), pj = pair ? axis_pos(4 * t, D, Bj) : AxisPos{0, 0};
+    const f4v a = __builtin_nontemporal_load(in + 2 * t), b = __builtin_nontemporal_load(in + 2 * t + 1);
+    C<float> x[4] = {mk<float>(a.x, a.y), mk<float>(a.z, a.w), mk<float>(b.x, b.y), mk<float>(b.z, b.w)};
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      x[k] = cmul(diag_factor(d, pi.i, pj.i, pair, conj), x[k]);
+      axis_next(pi, D, Bi);
+      if (pair) axis_next(pj, D, Bj);
+    }
+    __builtin_nontemporal_store(f4v{x[0].re, x[0].im, x[1].re, x[1].im}, out + 2 * t);
+    __builtin_nontemporal_store(f4v{x[2].re, x[2].im, x[3].re, x[3].im}, out + 2 * t + 1);
+  }
+}
+
 // two-sided diagonal gate on a DM: rho'[i, j] = d_i conj(d_j) rho[i, j]
 template <typename T>
 __global__ void __launch_bounds__(256)
@@ -246,6 +295,41 @@ k_pair_diag_adj(const C<T>* __restrict__ psi, const C<T>* __restrict__ lo_, C<T>
     const C<T> fx = cmul(f, psi[e]);
     const C<T> u = mk<T>(-w * fx.im, w * fx.re);
     g0 += (double)re_conj_mul(l, u);
+  }
+  write_partials(g0, 0.0, partial, nwg, zp1);
+}
+
+// adjoint: lam_in = conj(f) lam_out;  g += Re conj(lam_out) (i w f psi) with
+// w = sigma i (R), sigma i^2 (K) on one axis, i - j or i^2 - j^2 on a DM mode
+__global__ void __launch_bounds__(256)
+k_diag_adj_c64(const f4v* __restrict__ psi, const f4v* __restrict__ lo_, f4v* __restrict__ li, int64_t N4, int D,
+               int64_t Bi, int64_t Bj, const C<float>* __restrict__ d, int pair, int kerr, int conj, double* partial,
+               int nwg, int zp1) {
+  double g0 = 0.0;
+  GRID_STRIDE(t, N4) {
+    AxisPos pi = axis_pos(4 * t, D, Bi), pj = pair ? axis_pos(4 * t, D, Bj) : AxisPos{0, 0};
+    const f4v xa = __builtin_nontemporal_load(psi + 2 * t), xb = __builtin_nontemporal_load(psi + 2 * t + 1);
+    const f4v la = __builtin_nontemporal_load(lo_ + 2 * t), lb = __builtin_nontemporal_load(lo_ + 2 * t + 1);
+    const C<float> x[4] = {mk<float>(xa.x, xa.y), mk<float>(xa.z, xa.w), mk<float>(xb.x, xb.y), mk<float>(xb.z, xb.w)};
+    C<float> l[4] = {mk<float>(la.x, la.y), mk<float>(la.z, la.w), mk<float>(lb.x, lb.y), mk<float>(lb.z, lb.w)};
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const C<float> f = diag_factor(d, pi.i, pj.i, pair, conj);
+      float w;
+      if (pair)
+        w = kerr ? (float)(pi.i * pi.i - pj.i * pj.i) : (float)(pi.i - pj.i);
+      else
+        w = (float)(kerr ? pi.i * pi.i : pi.i) * (conj ? -1.f : 1.f);
+      const C<float> fx = cmul(f, x[k]);
+      s += re_conj_mul(l[k], mk<float>(-w * fx.im, w * fx.re));
+      l[k] = cmul(cconj(f), l[k]);
+      axis_next(pi, D, Bi);
+      if (pair) axis_next(pj, D, Bj);
+    }
+    g0 += (double)s;
+    __builtin_nontemporal_store(f4v{l[0].re, l[0].im, l[1].re, l[1].im}, li + 2 * t);
+    __builtin_nontemporal_store(f4v{l[2].re, l[2].im, l[3].re, l[3].im}, li + 2 * t + 1);
   }
   write_partials(g0, 0.0, partial, nwg, zp1);
 }
@@ -679,8 +763,24 @@ int axis_dense(const void* in, void* out, int64_t N, int D, int64_t B, const voi
   return (int)hipGetLastError();
 }
 
+// grid for the 4-amplitude complex64 diagonal kernels
+static bool diag4_ok(int64_t N, const void* a, const void* b, const void* c) {
+  return N % 4 == 0 && (((uintptr_t)a | (uintptr_t)b | (uintptr_t)c) & 15) == 0;
+}
+
+static int diag4_grid(int64_t N4) {
+  int64_t g = (N4 + kBlock - 1) / kBlock;
+  if (g > 4 * kMaxGrid) g = 4 * kMaxGrid;
+  return (int)(g < 1 ? 1 : g);
+}
+
 int axis_diag(const void* in, void* out, int64_t N, int D, int64_t B, const void* d, int conj, int dtype,
               hipStream_t s) {
+  if (dtype == SFAMD_C64 && diag4_ok(N, in, out, out)) {
+    hipLaunchKernelGGL(k_diag_c64, dim3(diag4_grid(N / 4)), dim3(kBlock), 0, s, (const f4v*)in, (f4v*)out, N / 4, D,
+                       B, B, CP(float, d), 0, conj);
+    return (int)hipGetLastError();
+  }
   if (dtype == SFAMD_C64)
     hipLaunchKernelGGL(k_axis_diag<float>, dim3(grid_for(N)), dim3(kBlock), 0, s, CP(float, in), MP(float, out), N,
                        D, B, CP(float, d), conj);
@@ -692,6 +792,11 @@ int axis_diag(const void* in, void* out, int64_t N, int D, int64_t B, const void
 
 int pair_diag(const void* in, void* out, int64_t N, int D, int64_t Bbra, int64_t Bket, const void* d, int dtype,
               hipStream_t s) {
+  if (dtype == SFAMD_C64 && diag4_ok(N, in, out, out)) {
+    hipLaunchKernelGGL(k_diag_c64, dim3(diag4_grid(N / 4)), dim3(kBlock), 0, s, (const f4v*)in, (f4v*)out, N / 4, D,
+                       Bbra, Bket, CP(float, d), 1, 0);
+    return (int)hipGetLastError();
+  }
   if (dtype == SFAMD_C64)
     hipLaunchKernelGGL(k_pair_diag<float>, dim3(grid_for(N)), dim3(kBlock), 0, s, CP(float, in), MP(float, out), N,
                        D, Bbra, Bket, CP(float, d));
@@ -765,6 +870,11 @@ int axis_dense_adj(const void* psi, const void* lam_out, void* lam_in, int64_t N
 
 int axis_diag_adj(const void* psi, const void* lam_out, void* lam_in, int64_t N, int D, int64_t B, const void* d,
                   int kerr, int conj, double* partial, int nwg, int zp1, int dtype, hipStream_t s) {
+  if (dtype == SFAMD_C64 && diag4_ok(N, psi, lam_out, lam_in)) {
+    hipLaunchKernelGGL(k_diag_adj_c64, dim3(nwg), dim3(kBlock), 0, s, (const f4v*)psi, (const f4v*)lam_out,
+                       (f4v*)lam_in, N / 4, D, B, B, CP(float, d), 0, kerr, conj, partial, nwg, zp1);
+    return (int)hipGetLastError();
+  }
   if (dtype == SFAMD_C64)
     hipLaunchKernelGGL(k_axis_diag_adj<float>, dim3(nwg), dim3(kBlock), 0, s, CP(float, psi), CP(float, lam_out),
                        MP(float, lam_in), N, D, B, CP(float, d), kerr, conj, partial, nwg, zp1);
@@ -776,6 +886,11 @@ int axis_diag_adj(const void* psi, const void* lam_out, void* lam_in, int64_t N,
 
 int pair_diag_adj(const void* psi, const void* lam_out, void* lam_in, int64_t N, int D, int64_t Bbra, int64_t Bket,
                   const void* d, int kerr, double* partial, int nwg, int zp1, int dtype, hipStream_t s) {
+  if (dtype == SFAMD_C64 && diag4_ok(N, psi, lam_out, lam_in)) {
+    hipLaunchKernelGGL(k_diag_adj_c64, dim3(nwg), dim3(kBlock), 0, s, (const f4v*)psi, (const f4v*)lam_out,
+                       (f4v*)lam_in, N / 4, D, Bbra, Bket, CP(float, d), 1, kerr, 0, partial, nwg, zp1);
+    return (int)hipGetLastError();
+  }
   if (dtype == SFAMD_C64)
     hipLaunchKernelGGL(k_pair_diag_adj<float>, dim3(nwg), dim3(kBlock), 0, s, CP(float, psi), CP(float, lam_out),
                        MP(float, lam_in), N, D, Bbra, Bket, CP(float, d), kerr, partial, nwg, zp1);

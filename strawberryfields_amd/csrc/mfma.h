@@ -23,4 +23,9 @@ int pair_bs_adj_mfma(const void* psi, const void* lam_out, void* lam_in, int64_t
                      const void* blocks, const void* dblocks, int conj, double* partial, int nwg, int zp1,
                      hipStream_t s);
 
+// rho' = U rho U^H on one mode of a DM (bra stride D * sk, ket stride sk) in
+// one pass; supported for sk == 1 or sk >= 16
+bool dm_dense_mfma_supported(int D, int dtype, int64_t sk);
+int dm_dense_mfma(const void* in, void* out, int64_t N, int D, int64_t sk, const void* M, hipStream_t s);
+
 }  // namespace sfamd

@@ -25,18 +25,11 @@
 // stride is >= 16, and 16 whole contiguous fibers when the axis is innermost.
 #include "common.h"
 #include "mfma.h"
+#include "mfma_ops.h"
 
 namespace {
 
-typedef float f4 __attribute__((ext_vector_type(4)));
-using C = cpx<float>;
-
-// Register plan for an operator of up to 8 MB complex rows and columns.
-template <int MB>
-struct Blk {
-  static constexpr int Q = 2 * MB;   // amplitudes per lane group
-  static constexpr int NS = 4 * MB;  // k-steps (4 real columns each)
-};
+using namespace sfamd::mf;
 
 template <int D>
 struct Plan {
@@ -45,92 +38,15 @@ struct Plan {
   static constexpr int NS = Blk<Mb>::NS;
 };
 
-// this lane's A operands of the real form of a complex operator given by
-// entry(i, k) (row i = output amplitude, column k = input amplitude; zero
-// outside the operator): A[mb][s]
-template <int MB, typename F>
-__device__ __forceinline__ void load_op(F entry, float (&A)[MB][Blk<MB>::NS]) {
-  constexpr int Q = Blk<MB>::Q;
-  const int lane = threadIdx.x & 63;
-  const int c = lane & 15, g = lane >> 4;
-#pragma unroll
-  for (int mb = 0; mb < MB; ++mb)
-#pragma unroll
-    for (int s = 0; s < Blk<MB>::NS; ++s) {
-      const int a = c & 1, b = s & 1;
-      const C m = entry((c >> 2) * Q + 2 * mb + ((c & 3) >> 1), g * Q + (s >> 1));
-      A[mb][s] = a == b ? m.re : (a ? m.im : -m.im);
-    }
-}
-
-// entry of op(M) for an n x n row-major matrix; op: 0 -> M, 1 -> conj(M),
-// 2 -> M^H, 3 -> M^T
-__device__ __forceinline__ C op_entry(const C* __restrict__ M, int n, int op, int i, int k) {
-  if (i >= n || k >= n) return mk<float>(0.f, 0.f);
-  C m = op >= 2 ? M[k * n + i] : M[i * n + k];
-  if (op == 1 || op == 2) m.im = -m.im;
-  return m;
-}
-
 template <int D>
 __device__ __forceinline__ void load_operator(const C* __restrict__ M, int op, float (&A)[Plan<D>::Mb][Plan<D>::NS]) {
   load_op<Plan<D>::Mb>([&](int i, int k) { return op_entry(M, D, op, i, k); }, A);
-}
-
-// acc = op x for this lane's 16-fiber block (x in group layout)
-template <int MB>
-__device__ __forceinline__ void apply_op(const float (&A)[MB][Blk<MB>::NS], const C (&x)[Blk<MB>::Q], f4 (&acc)[MB]) {
-#pragma unroll
-  for (int mb = 0; mb < MB; ++mb) acc[mb] = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for (int s = 0; s < Blk<MB>::NS; ++s) {
-    const float b = (s & 1) ? x[s >> 1].im : x[s >> 1].re;
-#pragma unroll
-    for (int mb = 0; mb < MB; ++mb) acc[mb] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[mb][s], b, acc[mb], 0, 0, 0);
-  }
-}
-
-// Operators shared by every wave of a workgroup (adjoint kernels) live in LDS
-// in lane order, dst[(mb * NS + s) * 64 + lane]; the block's threads build them
-// together and the A operand of each MFMA is one conflict-free ds_read.
-template <int MB, typename F>
-__device__ __forceinline__ void build_op_lds(F entry, float* dst) {
-  constexpr int Q = Blk<MB>::Q, NS = Blk<MB>::NS;
-  for (int e = threadIdx.x; e < MB * NS * 64; e += blockDim.x) {
-    const int lane = e & 63, slot = e >> 6;
-    const int mb = slot / NS, s = slot - mb * NS;
-    const int c = lane & 15, g = lane >> 4;
-    const int a = c & 1, b = s & 1;
-    const C m = entry((c >> 2) * Q + 2 * mb + ((c & 3) >> 1), g * Q + (s >> 1));
-    dst[e] = a == b ? m.re : (a ? m.im : -m.im);
-  }
-}
-
-template <int MB>
-__device__ __forceinline__ void apply_op_lds(const float* opl, const C (&x)[Blk<MB>::Q], f4 (&acc)[MB]) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int mb = 0; mb < MB; ++mb) acc[mb] = f4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for (int s = 0; s < Blk<MB>::NS; ++s) {
-    const float b = (s & 1) ? x[s >> 1].im : x[s >> 1].re;
-#pragma unroll
-    for (int mb = 0; mb < MB; ++mb)
-      acc[mb] = __builtin_amdgcn_mfma_f32_16x16x4f32(opl[(mb * Blk<MB>::NS + s) * 64 + lane], b, acc[mb], 0, 0, 0);
-  }
 }
 
 template <int D>
 __device__ __forceinline__ void apply_operator(const float (&A)[Plan<D>::Mb][Plan<D>::NS], const C (&x)[Plan<D>::Q],
                                                f4 (&acc)[Plan<D>::Mb]) {
   apply_op<Plan<D>::Mb>(A, x, acc);
-}
-
-// amplitude q of this lane's group from the accumulators
-template <int MB>
-__device__ __forceinline__ C acc_at(const f4 (&acc)[MB], int q) {
-  const f4 v = acc[q >> 1];
-  return (q & 1) ? mk<float>(v[2], v[3]) : mk<float>(v[0], v[1]);
 }
 
 template <int D>
@@ -410,6 +326,106 @@ k_bs_mfma_adj(const C* __restrict__ psi, const C* __restrict__ lo_, C* __restric
   mfma_write_partials(g0, g1, partial, nwg, zp1);
 }
 
+// ------------------------------------------------------------------ two-sided dense gate on a DM
+// rho' = U rho U^H on the (bra, ket) axes of one mode (strides sb = D sk, sk)
+// in one HBM pass (fockbackend applies U rho_blk U^H per slice,
+// fockbackend/circuit.py:179-183,212-214).  A workgroup takes 16 "columns" tau
+// of D x D blocks X_tau[i][j] and applies U along i (fibers (tau, j)), keeps
+// Y = U X in LDS, then applies conj(U) along j (fibers (tau, i)).
+//   sk >= 16: tau = 16 consecutive inner positions; both passes read and write
+//             HBM directly in 128-byte runs, Y lives in LDS as [i][j][tau].
+//   sk == 1:  tau = 16 consecutive D x D slabs (contiguous); the slabs are
+//             staged in LDS ([tau][i][j], slab stride D*D + 1 against bank
+//             conflicts) and both passes run in place.
+template <int D, bool STAGED>
+__global__ void __launch_bounds__(256)
+k_dm_dense_mfma(const C* __restrict__ in, C* __restrict__ out, int64_t N, int64_t sk, const C* __restrict__ M) {
+  constexpr int MB = Plan<D>::Mb, Q = Plan<D>::Q;
+  constexpr int PT = D * D + 1;
+  __shared__ C sh[STAGED ? 16 * PT : D * D * 16];
+  float AU[MB][Plan<D>::NS], AC[MB][Plan<D>::NS];
+  load_operator<D>(M, 0, AU);
+  load_operator<D>(M, 1, AC);
+  const int lane = threadIdx.x & 63;
+  const int c = lane & 15, g = lane >> 4;
+  const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int64_t sb = D * sk;
+  const int64_t nchunk = STAGED ? 1 : (sk + 15) / 16;
+  const int64_t nouter = N / ((int64_t)D * D * sk);
+  const int64_t ntile = STAGED ? (nouter + 15) / 16 : nouter * nchunk;
+  for (int64_t tile = blockIdx.x; tile < ntile; tile += gridDim.x) {
+    if (STAGED) {
+      const int64_t a0 = tile * 16;
+      const int gs = (int)(nouter - a0 < 16 ? nouter - a0 : 16);
+      const C* src = in + a0 * D * D;
+      for (int e = threadIdx.x; e < gs * D * D; e += blockDim.x) sh[(e / (D * D)) * PT + e % (D * D)] = src[e];
+      __syncthreads();
+      // pass 1: fibers (tau, j), amplitudes i; pass 2: fibers (tau, i), amplitudes j
+#pragma unroll
+      for (int pass = 0; pass < 2; ++pass) {
+        const int nblk = (gs * D + 15) / 16;
+        for (int b = wave; b < nblk; b += nw) {
+          const int phi = b * 16 + c;
+          const bool valid = phi < gs * D;
+          const int tau = valid ? phi / D : 0, r = valid ? phi % D : 0;
+          const int base = tau * PT + (pass == 0 ? r : r * D);
+          const int st = pass == 0 ? D : 1;
+          C x[Q];
+#pragma unroll
+          for (int q = 0; q < Q; ++q)
+            x[q] = (valid && g * Q + q < D) ? sh[base + (g * Q + q) * st] : mk<float>(0.f, 0.f);
+          f4 acc[MB];
+          if (pass == 0)
+            apply_op<MB>(AU, x, acc);
+          else
+            apply_op<MB>(AC, x, acc);
+          if (valid) {
+#pragma unroll
+            for (int q = 0; q < Q; ++q)
+              if (g * Q + q < D) sh[base + (g * Q + q) * st] = acc_at<MB>(acc, q);
+          }
+        }
+        __syncthreads();
+      }
+      C* dst = out + a0 * D * D;
+      for (int e = threadIdx.x; e < gs * D * D; e += blockDim.x) dst[e] = sh[(e / (D * D)) * PT + e % (D * D)];
+      __syncthreads();
+    } else {
+      const int64_t a = tile / nchunk;
+      const int64_t b0 = (tile - a * nchunk) * 16;
+      const bool valid = b0 + c < sk;
+      const int64_t col = a * D * sb + b0 + (valid ? c : 0);
+      // pass 1: block j, amplitudes i from HBM -> Y[i][j][tau] in LDS
+      for (int j = wave; j < D; j += nw) {
+        C x[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+          x[q] = (valid && g * Q + q < D) ? in[col + (g * Q + q) * sb + j * sk] : mk<float>(0.f, 0.f);
+        f4 acc[MB];
+        apply_op<MB>(AU, x, acc);
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+          if (g * Q + q < D) sh[((g * Q + q) * D + j) * 16 + c] = acc_at<MB>(acc, q);
+      }
+      __syncthreads();
+      // pass 2: block i, amplitudes j from LDS -> HBM
+      for (int i = wave; i < D; i += nw) {
+        C x[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) x[q] = g * Q + q < D ? sh[(i * D + g * Q + q) * 16 + c] : mk<float>(0.f, 0.f);
+        f4 acc[MB];
+        apply_op<MB>(AC, x, acc);
+        if (valid) {
+#pragma unroll
+          for (int q = 0; q < Q; ++q)
+            if (g * Q + q < D) out[col + i * sb + (g * Q + q) * sk] = acc_at<MB>(acc, q);
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
 #define SFAMD_MFMA_D(X) \
   X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18) X(19) X(20)
 
@@ -493,6 +509,33 @@ int pair_bs_adj_mfma(const void* psi, const void* lam_out, void* lam_in, int64_t
   case d:                                                                                                           \
     hipLaunchKernelGGL((k_bs_mfma_adj<d>), dim3(nwg), dim3(kBlock), 0, s, (const C*)psi, (const C*)lam_out,        \
                        (C*)lam_in, NF, g, sp, sq, (const C*)blocks, (const C*)dblocks, conj, partial, nwg, zp1);    \
+    break;
+    SFAMD_MFMA_D(CASE)
+#undef CASE
+    default:
+      return SFAMD_E_CUTOFF;
+  }
+  return (int)hipGetLastError();
+}
+
+bool dm_dense_mfma_supported(int D, int dtype, int64_t sk) {
+  return mfma_supported(D, dtype) && (sk == 1 || sk >= 16);
+}
+
+int dm_dense_mfma(const void* in, void* out, int64_t N, int D, int64_t sk, const void* M, hipStream_t s) {
+  if (!(sk == 1 || sk >= 16)) return SFAMD_E_ARG;
+  const int64_t nouter = N / ((int64_t)D * D * sk);
+  const int64_t ntile = sk == 1 ? (nouter + 15) / 16 : nouter * ((sk + 15) / 16);
+  const int grid = (int)(ntile < kMaxGrid ? ntile : kMaxGrid);
+  switch (D) {
+#define CASE(d)                                                                                              \
+  case d:                                                                                                    \
+    if (sk == 1)                                                                                             \
+      hipLaunchKernelGGL((k_dm_dense_mfma<d, true>), dim3(grid), dim3(kBlock), 0, s, (const C*)in, (C*)out, \
+                         N, sk, (const C*)M);                                                                \
+    else                                                                                                     \
+      hipLaunchKernelGGL((k_dm_dense_mfma<d, false>), dim3(grid), dim3(kBlock), 0, s, (const C*)in,         \
+                         (C*)out, N, sk, (const C*)M);                                                       \
     break;
     SFAMD_MFMA_D(CASE)
 #undef CASE

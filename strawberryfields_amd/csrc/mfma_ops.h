@@ -1,0 +1,109 @@
+// Matrix-core building blocks shared by the complex64 gate kernels (gfx950).
+//
+// A complex operator (out = Op x, Op given entry by entry) is applied to 16
+// fibers per wave with v_mfma_f32_16x16x4_f32 on its real form
+//   [out_re; out_im] = [Re Op, -Im Op; Im Op, Re Op] [x_re; x_im],
+// padded to 16 MB rows and columns.  The instruction's operand layouts fix the
+// register plan: lane l = 16 g + c supplies A[c][g] and B[g][c] and receives
+// C[4 g + j][c].  Columns c are fibers, so a lane touches amplitudes of one
+// fiber only, and group g owns amplitudes g Q .. g Q + Q - 1 (Q = 2 MB) of
+// both input and output:
+//   row 16 mb + c      <->  amplitude (c / 4) Q + 2 mb + (c % 4) / 2, part c % 2
+//   k-step s, group g  <->  amplitude g Q + s / 2, part s % 2.
+#pragma once
+
+#include "common.h"
+
+namespace sfamd {
+namespace mf {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+using C = cpx<float>;
+
+
+// Register plan for an operator of up to 8 MB complex rows and columns.
+template <int MB>
+struct Blk {
+  static constexpr int Q = 2 * MB;   // amplitudes per lane group
+  static constexpr int NS = 4 * MB;  // k-steps (4 real columns each)
+};
+
+// this lane's A operands of the real form of a complex operator given by
+// entry(i, k) (row i = output amplitude, column k = input amplitude; zero
+// outside the operator): A[mb][s]
+template <int MB, typename F>
+__device__ __forceinline__ void load_op(F entry, float (&A)[MB][Blk<MB>::NS]) {
+  constexpr int Q = Blk<MB>::Q;
+  const int lane = threadIdx.x & 63;
+  const int c = lane & 15, g = lane >> 4;
+#pragma unroll
+  for (int mb = 0; mb < MB; ++mb)
+#pragma unroll
+    for (int s = 0; s < Blk<MB>::NS; ++s) {
+      const int a = c & 1, b = s & 1;
+      const C m = entry((c >> 2) * Q + 2 * mb + ((c & 3) >> 1), g * Q + (s >> 1));
+      A[mb][s] = a == b ? m.re : (a ? m.im : -m.im);
+    }
+}
+
+// entry of op(M) for an n x n row-major matrix; op: 0 -> M, 1 -> conj(M),
+// 2 -> M^H, 3 -> M^T
+__device__ __forceinline__ C op_entry(const C* __restrict__ M, int n, int op, int i, int k) {
+  if (i >= n || k >= n) return mk<float>(0.f, 0.f);
+  C m = op >= 2 ? M[k * n + i] : M[i * n + k];
+  if (op == 1 || op == 2) m.im = -m.im;
+  return m;
+}
+
+// acc = op x for this lane's 16-fiber block (x in group layout)
+template <int MB>
+__device__ __forceinline__ void apply_op(const float (&A)[MB][Blk<MB>::NS], const C (&x)[Blk<MB>::Q], f4 (&acc)[MB]) {
+#pragma unroll
+  for (int mb = 0; mb < MB; ++mb) acc[mb] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int s = 0; s < Blk<MB>::NS; ++s) {
+    const float b = (s & 1) ? x[s >> 1].im : x[s >> 1].re;
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb) acc[mb] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[mb][s], b, acc[mb], 0, 0, 0);
+  }
+}
+
+// Operators shared by every wave of a workgroup (adjoint kernels) live in LDS
+// in lane order, dst[(mb * NS + s) * 64 + lane]; the block's threads build them
+// together and the A operand of each MFMA is one conflict-free ds_read.
+template <int MB, typename F>
+__device__ __forceinline__ void build_op_lds(F entry, float* dst) {
+  constexpr int Q = Blk<MB>::Q, NS = Blk<MB>::NS;
+  for (int e = threadIdx.x; e < MB * NS * 64; e += blockDim.x) {
+    const int lane = e & 63, slot = e >> 6;
+    const int mb = slot / NS, s = slot - mb * NS;
+    const int c = lane & 15, g = lane >> 4;
+    const int a = c & 1, b = s & 1;
+    const C m = entry((c >> 2) * Q + 2 * mb + ((c & 3) >> 1), g * Q + (s >> 1));
+    dst[e] = a == b ? m.re : (a ? m.im : -m.im);
+  }
+}
+
+template <int MB>
+__device__ __forceinline__ void apply_op_lds(const float* opl, const C (&x)[Blk<MB>::Q], f4 (&acc)[MB]) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int mb = 0; mb < MB; ++mb) acc[mb] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int s = 0; s < Blk<MB>::NS; ++s) {
+    const float b = (s & 1) ? x[s >> 1].im : x[s >> 1].re;
+#pragma unroll
+    for (int mb = 0; mb < MB; ++mb)
+      acc[mb] = __builtin_amdgcn_mfma_f32_16x16x4f32(opl[(mb * Blk<MB>::NS + s) * 64 + lane], b, acc[mb], 0, 0, 0);
+  }
+}
+
+// amplitude q of this lane's group from the accumulators
+template <int MB>
+__device__ __forceinline__ C acc_at(const f4 (&acc)[MB], int q) {
+  const f4 v = acc[q >> 1];
+  return (q & 1) ? mk<float>(v[2], v[3]) : mk<float>(v[0], v[1]);
+}
+
+}  // namespace mf
+}  // namespace sfamd

@@ -133,6 +133,8 @@ int apply_op(const OpPlan& op, int n, int D, const void* table, const void* in, 
     case SFAMD_OP_SQUEEZE: {
       const int par = op.kind == SFAMD_OP_SQUEEZE;
       if (op.mixed_in) {
+        const int64_t sk = ket_stride_dm(n, D, op.m0);
+        if (dm_dense_mfma_supported(D, dtype, sk)) return dm_dense_mfma(in, out, NM, D, sk, g, s);
         if (!tmp) return SFAMD_E_WORKSPACE;
         RET_IF(dense_pass(in, tmp, NM, D, bra_stride_dm(n, D, op.m0), g, 0, par, dtype, s));
         return dense_pass(tmp, out, NM, D, ket_stride_dm(n, D, op.m0), g, 1, par, dtype, s);

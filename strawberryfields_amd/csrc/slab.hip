@@ -20,6 +20,7 @@
 // Gate coefficients (beamsplitter blocks, loss coefficients) are staged in LDS
 // as well when they fit, since lanes of one wave work on different blocks.
 #include "common.h"
+#include "mfma_ops.h"
 #include "slab.h"
 
 namespace {
@@ -63,13 +64,70 @@ __device__ __forceinline__ int64_t tile_off(const Tile& t, const View& v, int e,
   return gsl * t.slab + (int64_t)(j - gsl * D) * t.S_out + r;
 }
 
+// Tile copies move 16-byte chunks, eight nontemporal loads in flight per
+// thread before the LDS writes.  A tile is gs * D runs of v.run contiguous
+// amplitudes (run j = slab * D + outer-axis position starts at
+// (j / D) * slab + (j % D) * S_out); tiles whose runs are not whole 16-byte
+// chunks fall back to amplitude-by-amplitude copies.
+typedef float f4s __attribute__((ext_vector_type(4)));
+constexpr int kCopyBatch = 8;
+
+template <typename T>
+__device__ __forceinline__ bool tile_chunked(const Tile& t, const View& v, const C<T>* g) {
+  const uintptr_t esz = sizeof(C<T>);
+  return ((((uintptr_t)(g + v.base)) | (uintptr_t)(v.run * esz) | (uintptr_t)(t.S_out * esz)) & 15) == 0;
+}
+
+// byte offset from g + v.base of 16-byte chunk c (cpr chunks per run);
+// cpr == 0 marks a contiguous tile
+template <typename T>
+__device__ __forceinline__ int64_t chunk_off(const Tile& t, int c, int cpr, int D) {
+  if (cpr == 0) return (int64_t)c * 16;
+  const int j = c / cpr;
+  const int r = c - j * cpr;
+  const int gsl = j / D;
+  return ((int64_t)gsl * t.slab + (int64_t)(j - gsl * D) * t.S_out) * (int64_t)sizeof(C<T>) + (int64_t)r * 16;
+}
+
 template <typename T>
 __device__ __forceinline__ void tile_load(C<T>* sh, const C<T>* __restrict__ g, const Tile& t, const View& v, int D) {
+  if (tile_chunked(t, v, g)) {
+    const char* src = (const char*)(g + v.base);
+    f4s* dst = (f4s*)sh;
+    const int cpr = v.run * D == t.slab ? 0 : (int)(v.run * sizeof(C<T>) / 16);
+    const int n = (int)(v.elems * sizeof(C<T>) / 16);
+    for (int c0 = threadIdx.x; c0 < n; c0 += kCopyBatch * blockDim.x) {
+      f4s r[kCopyBatch];
+#pragma unroll
+      for (int k = 0; k < kCopyBatch; ++k) {
+        const int c = c0 + k * (int)blockDim.x;
+        if (c < n) r[k] = __builtin_nontemporal_load((const f4s*)(src + chunk_off<T>(t, c, cpr, D)));
+      }
+#pragma unroll
+      for (int k = 0; k < kCopyBatch; ++k)
+        if (c0 + k * (int)blockDim.x < n) dst[c0 + k * blockDim.x] = r[k];
+    }
+    return;
+  }
   for (int e = threadIdx.x; e < v.elems; e += blockDim.x) sh[e] = g[v.base + tile_off(t, v, e, D)];
 }
 
 template <typename T>
 __device__ __forceinline__ void tile_store(const C<T>* sh, C<T>* __restrict__ g, const Tile& t, const View& v, int D) {
+  if (tile_chunked(t, v, (const C<T>*)g)) {
+    char* dst = (char*)(g + v.base);
+    const f4s* src = (const f4s*)sh;
+    const int cpr = v.run * D == t.slab ? 0 : (int)(v.run * sizeof(C<T>) / 16);
+    const int n = (int)(v.elems * sizeof(C<T>) / 16);
+    for (int c0 = threadIdx.x; c0 < n; c0 += kCopyBatch * blockDim.x) {
+#pragma unroll
+      for (int k = 0; k < kCopyBatch; ++k) {
+        const int c = c0 + k * (int)blockDim.x;
+        if (c < n) __builtin_nontemporal_store(src[c], (f4s*)(dst + chunk_off<T>(t, c, cpr, D)));
+      }
+    }
+    return;
+  }
   for (int e = threadIdx.x; e < v.elems; e += blockDim.x) g[v.base + tile_off(t, v, e, D)] = sh[e];
 }
 
@@ -379,6 +437,242 @@ k_loss_slab_adj(const C<T>* __restrict__ psi, const C<T>* __restrict__ lo_, C<T>
   write_partials(g0, 0.0, partial, nwg, zp1);
 }
 
+// ------------------------------------------------------------------ matrix-core variants (complex64)
+// The same tiles, with each block (beamsplitter photon number s, loss
+// diagonal d) applied as a real-form MFMA operator: a wave takes whole blocks,
+// builds the block's operator once and applies it to 16 fibers of the tile per
+// chain, reading and writing the tile in place.  A policy describes the gate:
+// block count, block size, fiber positions and operator entries.
+using sfamd::mf::Blk;
+using sfamd::mf::f4;
+using Cf = cpx<float>;
+
+template <int D>
+struct BsPol {
+  static constexpr int NB = 2 * D - 1;
+  static constexpr bool kTwoParams = true;
+  const Cf* blk;
+  const Cf* dblk;
+  int conj, p_outer;
+  // gate table (blocks, and derivative blocks for the adjoint) copied to LDS
+  static constexpr int64_t table_bytes(bool adj) { return (adj ? 2 : 1) * bs_nb(D) * (int64_t)sizeof(Cf); }
+  __device__ BsPol staged(unsigned char* lds, bool adj) const {
+    BsPol p = *this;
+    Cf* m = (Cf*)lds;
+    stage(m, blk, bs_nb(D));
+    p.blk = m;
+    if (adj) {
+      stage(m + bs_nb(D), dblk, bs_nb(D));
+      p.dblk = m + bs_nb(D);
+    }
+    return p;
+  }
+  __device__ int size(int s) const { return bs_hi(s, D) - bs_lo(s, D) + 1; }
+  __device__ int per_block(const Tile& t, const View& v) const { return v.gs * v.mt * t.Sin; }
+  __device__ void fiber(int s, int phi, const Tile& t, const View& v, int& pos0, int& step) const {
+    const BsFiber f = bs_fiber(s * per_block(t, v) + phi, per_block(t, v), t, v, p_outer, D);
+    pos0 = f.pos0;
+    step = f.step;
+  }
+  __device__ Cf entry(int s, int n, int i, int k) const {
+    return sfamd::mf::op_entry(blk + bs_block_off(s, D), n, conj ? 1 : 0, i, k);
+  }
+  __device__ Cf entry_h(int s, int n, int i, int k) const {
+    return sfamd::mf::op_entry(blk + bs_block_off(s, D), n, conj ? 3 : 2, i, k);
+  }
+  __device__ Cf dentry0(int s, int n, int i, int k) const {
+    return sfamd::mf::op_entry(dblk + bs_block_off(s, D), n, conj ? 1 : 0, i, k);
+  }
+  // d/dphi of B'[N][n] = i sg (n - N) B'[N][n], sg = -1 on the conjugated side
+  __device__ Cf dentry1(int s, int n, int i, int k) const {
+    const Cf m = entry(s, n, i, k);
+    const float w = (conj ? -1.f : 1.f) * (float)(k - i);
+    return mk<float>(-w * m.im, w * m.re);
+  }
+};
+
+template <int D>
+struct LossPol {
+  static constexpr int NB = 2 * D - 1;
+  static constexpr bool kTwoParams = false;
+  const float* coef;
+  const float* dcoef;
+  static constexpr int64_t table_bytes(bool adj) { return (adj ? 2 : 1) * (int64_t)D * D * D * sizeof(float); }
+  __device__ LossPol staged(unsigned char* lds, bool adj) const {
+    LossPol p = *this;
+    float* m = (float*)lds;
+    stage(m, coef, (int64_t)D * D * D);
+    p.coef = m;
+    if (adj) {
+      stage(m + D * D * D, dcoef, (int64_t)D * D * D);
+      p.dcoef = m + D * D * D;
+    }
+    return p;
+  }
+  __device__ int size(int di) const {
+    const int d = di - (D - 1);
+    return D - (d < 0 ? -d : d);
+  }
+  __device__ int per_block(const Tile& t, const View& v) const { return v.gs * t.Sin; }
+  __device__ void fiber(int di, int phi, const Tile& t, const View& v, int& pos0, int& step) const {
+    const LossFiber f = loss_fiber(di * per_block(t, v) + phi, per_block(t, v), t, v, D);
+    pos0 = f.pos0;
+    step = f.step;
+  }
+  // out[i] = sum_l c[p0 + i][p0 + i + d][l] x[i + l]: entry (i, k = i + l)
+  __device__ Cf coef_entry(const float* cf, int di, int n, int i, int k) const {
+    if (i >= n || k >= n || k < i) return mk<float>(0.f, 0.f);
+    const int d = di - (D - 1);
+    const int p0 = d < 0 ? -d : 0;
+    return mk<float>(cf[((p0 * D) + p0 + d) * D + i * (D + 1) * D + (k - i)], 0.f);
+  }
+  __device__ Cf entry(int di, int n, int i, int k) const { return coef_entry(coef, di, n, i, k); }
+  __device__ Cf entry_h(int di, int n, int i, int k) const { return coef_entry(coef, di, n, k, i); }
+  __device__ Cf dentry0(int di, int n, int i, int k) const { return coef_entry(dcoef, di, n, i, k); }
+  __device__ Cf dentry1(int, int, int, int) const { return mk<float>(0.f, 0.f); }
+};
+
+template <int MB, int D, class P>
+__device__ __forceinline__ void slab_block(Cf* sh, const P& pol, int b, int n, const Tile& t, const View& v) {
+  constexpr int Q = Blk<MB>::Q;
+  float A[MB][Blk<MB>::NS];
+  sfamd::mf::load_op<MB>([&](int i, int k) { return pol.entry(b, n, i, k); }, A);
+  const int per = pol.per_block(t, v);
+  const int lane = threadIdx.x & 63;
+  const int cl = lane & 15, g = lane >> 4;
+  for (int fb = 0; fb * 16 < per; ++fb) {
+    const int phi = fb * 16 + cl;
+    const bool valid = phi < per;
+    int pos0, step;
+    pol.fiber(b, valid ? phi : 0, t, v, pos0, step);
+    pos0 += g * Q * step;
+    Cf x[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) x[q] = (valid && g * Q + q < n) ? sh[pos0 + q * step] : mk<float>(0.f, 0.f);
+    f4 acc[MB];
+    sfamd::mf::apply_op<MB>(A, x, acc);
+    if (valid) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q)
+        if (g * Q + q < n) sh[pos0 + q * step] = sfamd::mf::acc_at<MB>(acc, q);
+    }
+  }
+}
+
+template <int D, class P>
+__global__ void __launch_bounds__(256) k_slab_mfma(const Cf* __restrict__ in, Cf* __restrict__ out, Tile t, P gpol) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  Cf* sh = (Cf*)smem;
+  const P pol = gpol.staged((unsigned char*)(sh + t.cap), false);  // synced by the first tile's barrier
+  const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int64_t tile = blockIdx.x; tile < t.ntile; tile += gridDim.x) {
+    const View v = tile_view(t, tile, D);
+    tile_load(sh, in, t, v, D);
+    __syncthreads();
+    for (int b = wave; b < P::NB; b += nw) {
+      const int n = pol.size(b);
+      const int mb = (2 * n + 15) / 16;
+      if (mb == 1)
+        slab_block<1, D>(sh, pol, b, n, t, v);
+      else if constexpr (D > 8) {
+        if (mb == 2)
+          slab_block<2, D>(sh, pol, b, n, t, v);
+        else if constexpr (D > 16)
+          slab_block<3, D>(sh, pol, b, n, t, v);
+      }
+    }
+    __syncthreads();
+    tile_store(sh, out, t, v, D);
+    __syncthreads();
+  }
+}
+
+// one sweep of a block's fibers with one operator: kind 0/1 accumulate the
+// gradient Re conj(lam) (A x) into g; kind 2 writes lam_in = A lam in place
+template <int MB, int D, class P, typename F>
+__device__ __forceinline__ void slab_sweep_adj(const Cf* shx, Cf* shl, const P& pol, int b, int n, const Tile& t,
+                                               const View& v, F entry, int kind, double& g) {
+  constexpr int Q = Blk<MB>::Q;
+  float A[MB][Blk<MB>::NS];
+  sfamd::mf::load_op<MB>(entry, A);
+  const int per = pol.per_block(t, v);
+  const int lane = threadIdx.x & 63;
+  const int cl = lane & 15, gq = (lane >> 4) * Q;
+  float sum = 0.f;
+  for (int fb = 0; fb * 16 < per; ++fb) {
+    const int phi = fb * 16 + cl;
+    const bool valid = phi < per;
+    int pos0, step;
+    pol.fiber(b, valid ? phi : 0, t, v, pos0, step);
+    pos0 += gq * step;
+    Cf x[Q], l[Q];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const bool in_range = valid && gq + q < n;
+      l[q] = in_range ? shl[pos0 + q * step] : mk<float>(0.f, 0.f);
+      if (kind != 2) x[q] = in_range ? shx[pos0 + q * step] : mk<float>(0.f, 0.f);
+    }
+    f4 acc[MB];
+    if (kind == 2) {
+      sfamd::mf::apply_op<MB>(A, l, acc);
+      if (valid) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+          if (gq + q < n) shl[pos0 + q * step] = sfamd::mf::acc_at<MB>(acc, q);
+      }
+    } else {
+      sfamd::mf::apply_op<MB>(A, x, acc);
+#pragma unroll
+      for (int q = 0; q < Q; ++q) sum += re_conj_mul(l[q], sfamd::mf::acc_at<MB>(acc, q));
+    }
+  }
+  g += (double)sum;
+}
+
+// adjoint of one block: gradient sweeps first (they read lam), then lam_in
+template <int MB, int D, class P>
+__device__ __forceinline__ void slab_block_adj(const Cf* shx, Cf* shl, const P& pol, int b, int n, const Tile& t,
+                                               const View& v, double& g0, double& g1) {
+  slab_sweep_adj<MB, D>(shx, shl, pol, b, n, t, v, [&](int i, int k) { return pol.dentry0(b, n, i, k); }, 0, g0);
+  if (P::kTwoParams)
+    slab_sweep_adj<MB, D>(shx, shl, pol, b, n, t, v, [&](int i, int k) { return pol.dentry1(b, n, i, k); }, 1, g1);
+  slab_sweep_adj<MB, D>(shx, shl, pol, b, n, t, v, [&](int i, int k) { return pol.entry_h(b, n, i, k); }, 2, g0);
+}
+
+template <int D, class P>
+__global__ void __launch_bounds__(512)
+k_slab_mfma_adj(const Cf* __restrict__ psi, const Cf* __restrict__ lo_, Cf* __restrict__ li, Tile t, P gpol,
+                double* partial, int nwg, int zp1) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  Cf* shx = (Cf*)smem;
+  Cf* shl = shx + t.cap;
+  const P pol = gpol.staged((unsigned char*)(shl + t.cap), true);  // synced by the first tile's barrier
+  const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  double g0 = 0.0, g1 = 0.0;
+  for (int64_t tile = blockIdx.x; tile < t.ntile; tile += gridDim.x) {
+    const View v = tile_view(t, tile, D);
+    tile_load(shx, psi, t, v, D);
+    tile_load(shl, lo_, t, v, D);
+    __syncthreads();
+    for (int b = wave; b < P::NB; b += nw) {
+      const int n = pol.size(b);
+      const int mb = (2 * n + 15) / 16;
+      if (mb == 1)
+        slab_block_adj<1, D>(shx, shl, pol, b, n, t, v, g0, g1);
+      else if constexpr (D > 8) {
+        if (mb == 2)
+          slab_block_adj<2, D>(shx, shl, pol, b, n, t, v, g0, g1);
+        else if constexpr (D > 16)
+          slab_block_adj<3, D>(shx, shl, pol, b, n, t, v, g0, g1);
+      }
+    }
+    __syncthreads();
+    tile_store(shl, li, t, v, D);
+    __syncthreads();
+  }
+  write_partials(g0, g1, partial, nwg, zp1);
+}
+
 // ------------------------------------------------------------------ host side
 #define SFAMD_SLAB_D(X) \
   X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18) X(19) X(20)
@@ -407,6 +701,19 @@ bool plan_tile(Tile& t, int64_t N, int D, int64_t S_out, int64_t Sin, int64_t li
   }
   t.cap = t.G * D * t.Mt * t.RW;
   t.ntile = (t.nslab + t.G - 1) / t.G * t.nmc;
+  return true;
+}
+
+// matrix-core tiles: 4096 amplitudes per array (leaves LDS for the staged gate
+// table), whole fiber blocks of 16 when a tile holds >= 16 slabs
+constexpr int64_t kMfmaTile = 4096;
+bool plan_tile_mfma(Tile& t, int64_t N, int D, int64_t S_out, int64_t Sin, int64_t limit) {
+  if (!plan_tile(t, N, D, S_out, Sin, limit)) return false;
+  if (t.G >= 16) {
+    t.G -= t.G % 16;
+    t.cap = t.G * D * t.Mt * t.RW;
+    t.ntile = (t.nslab + t.G - 1) / t.G * t.nmc;
+  }
   return true;
 }
 
@@ -511,8 +818,99 @@ static int loss_slab_adj_launch(const void* psi, const void* lo, void* li, const
   return (int)hipGetLastError();
 }
 
+// ---- complex64: matrix-core slab kernels
+template <int D, class P>
+static int slab_mfma_launch(const void* in, void* out, const Tile& t, const P& pol, hipStream_t s) {
+  const size_t lds = (size_t)t.cap * sizeof(Cf) + P::table_bytes(false);
+  RET_IF(allow_lds(k_slab_mfma<D, P>, lds));
+  hipLaunchKernelGGL((k_slab_mfma<D, P>), dim3(slab_grid(t)), dim3(256), lds, s, (const Cf*)in, (Cf*)out, t, pol);
+  return (int)hipGetLastError();
+}
+
+template <int D, class P>
+static int slab_mfma_adj_launch(const void* psi, const void* lo, void* li, const Tile& t, const P& pol,
+                                double* partial, int nwg, int zp1, hipStream_t s) {
+  const size_t lds = 2 * (size_t)t.cap * sizeof(Cf) + P::table_bytes(true);
+  RET_IF(allow_lds(k_slab_mfma_adj<D, P>, lds));
+  hipLaunchKernelGGL((k_slab_mfma_adj<D, P>), dim3(nwg), dim3(512), lds, s, (const Cf*)psi, (const Cf*)lo, (Cf*)li,
+                     t, pol, partial, nwg, zp1);
+  return (int)hipGetLastError();
+}
+
+static int bs_slab_mfma(const void* in, void* out, int64_t N, int D, int64_t sp, int64_t sq, const void* blocks,
+                        int conj, hipStream_t s) {
+  Tile t;
+  const int p_outer = sp > sq;
+  if (!plan_tile_mfma(t, N, D, p_outer ? sp : sq, p_outer ? sq : sp, kMfmaTile)) return SFAMD_E_ARG;
+  switch (D) {
+#define CASE(d)                                                     \
+  case d: {                                                         \
+    BsPol<d> pol{(const Cf*)blocks, nullptr, conj, p_outer};        \
+    return slab_mfma_launch<d>(in, out, t, pol, s);                 \
+  }
+    SFAMD_SLAB_D(CASE)
+#undef CASE
+    default:
+      return SFAMD_E_CUTOFF;
+  }
+}
+
+static int bs_slab_mfma_adj(const void* psi, const void* lo, void* li, int64_t N, int D, int64_t sp, int64_t sq,
+                            const void* blocks, const void* dblocks, int conj, double* partial, int nwg, int zp1,
+                            hipStream_t s) {
+  Tile t;
+  const int p_outer = sp > sq;
+  if (!plan_tile_mfma(t, N, D, p_outer ? sp : sq, p_outer ? sq : sp, kMfmaTile)) return SFAMD_E_ARG;
+  switch (D) {
+#define CASE(d)                                                                    \
+  case d: {                                                                        \
+    BsPol<d> pol{(const Cf*)blocks, (const Cf*)dblocks, conj, p_outer};            \
+    return slab_mfma_adj_launch<d>(psi, lo, li, t, pol, partial, nwg, zp1, s);     \
+  }
+    SFAMD_SLAB_D(CASE)
+#undef CASE
+    default:
+      return SFAMD_E_CUTOFF;
+  }
+}
+
+static int loss_slab_mfma(const void* in, void* out, int64_t N, int D, int64_t sb, int64_t sk, const void* coef,
+                          hipStream_t s) {
+  Tile t;
+  if (!plan_tile_mfma(t, N, D, sb, sk, kMfmaTile)) return SFAMD_E_ARG;
+  switch (D) {
+#define CASE(d)                                             \
+  case d: {                                                 \
+    LossPol<d> pol{(const float*)coef, nullptr};            \
+    return slab_mfma_launch<d>(in, out, t, pol, s);         \
+  }
+    SFAMD_SLAB_D(CASE)
+#undef CASE
+    default:
+      return SFAMD_E_CUTOFF;
+  }
+}
+
+static int loss_slab_mfma_adj(const void* psi, const void* lo, void* li, int64_t N, int D, int64_t sb, int64_t sk,
+                              const void* coef, const void* dcoef, double* partial, int nwg, int zp1, hipStream_t s) {
+  Tile t;
+  if (!plan_tile_mfma(t, N, D, sb, sk, kMfmaTile)) return SFAMD_E_ARG;
+  switch (D) {
+#define CASE(d)                                                                    \
+  case d: {                                                                        \
+    LossPol<d> pol{(const float*)coef, (const float*)dcoef};                       \
+    return slab_mfma_adj_launch<d>(psi, lo, li, t, pol, partial, nwg, zp1, s);     \
+  }
+    SFAMD_SLAB_D(CASE)
+#undef CASE
+    default:
+      return SFAMD_E_CUTOFF;
+  }
+}
+
 int pair_bs_slab(const void* in, void* out, int64_t N, int D, int64_t sp, int64_t sq, const void* blocks, int conj,
                  int dtype, hipStream_t s) {
+  if (dtype == SFAMD_C64) return bs_slab_mfma(in, out, N, D, sp, sq, blocks, conj, s);
   Tile t;
   const int p_outer = sp > sq;
   if (!plan_tile(t, N, D, p_outer ? sp : sq, p_outer ? sq : sp, slab_limit(dtype, 0))) return SFAMD_E_ARG;
@@ -531,6 +929,8 @@ int pair_bs_slab(const void* in, void* out, int64_t N, int D, int64_t sp, int64_
 int pair_bs_adj_slab(const void* psi, const void* lam_out, void* lam_in, int64_t N, int D, int64_t sp, int64_t sq,
                      const void* blocks, const void* dblocks, int conj, double* partial, int nwg, int zp1, int dtype,
                      hipStream_t s) {
+  if (dtype == SFAMD_C64)
+    return bs_slab_mfma_adj(psi, lam_out, lam_in, N, D, sp, sq, blocks, dblocks, conj, partial, nwg, zp1, s);
   Tile t;
   const int p_outer = sp > sq;
   if (!plan_tile(t, N, D, p_outer ? sp : sq, p_outer ? sq : sp, slab_limit(dtype, 1))) return SFAMD_E_ARG;
@@ -550,6 +950,7 @@ int pair_bs_adj_slab(const void* psi, const void* lam_out, void* lam_in, int64_t
 
 int loss_slab(const void* in, void* out, int64_t N, int D, int64_t sb, int64_t sk, const void* coef, int dtype,
               hipStream_t s) {
+  if (dtype == SFAMD_C64) return loss_slab_mfma(in, out, N, D, sb, sk, coef, s);
   Tile t;
   if (!plan_tile(t, N, D, sb, sk, slab_limit(dtype, 0))) return SFAMD_E_ARG;
   switch (D) {
@@ -566,6 +967,7 @@ int loss_slab(const void* in, void* out, int64_t N, int D, int64_t sb, int64_t s
 
 int loss_adj_slab(const void* psi, const void* lam_out, void* lam_in, int64_t N, int D, int64_t sb, int64_t sk,
                   const void* coef, const void* dcoef, double* partial, int nwg, int zp1, int dtype, hipStream_t s) {
+  if (dtype == SFAMD_C64) return loss_slab_mfma_adj(psi, lam_out, lam_in, N, D, sb, sk, coef, dcoef, partial, nwg, zp1, s);
   Tile t;
   if (!plan_tile(t, N, D, sb, sk, slab_limit(dtype, 1))) return SFAMD_E_ARG;
   switch (D) {
