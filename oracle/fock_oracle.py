@@ -411,9 +411,19 @@ def all_fock_probs(state, n, pure):
 
 
 def reduced_dm(state, n, pure, modes):
-    """Reduced DM of sorted `modes`, interleaved axes (states.py:613-642)."""
-    rho = dm_of(state, n, pure)
+    """Reduced DM of sorted `modes`, interleaved axes (states.py:613-642).
+
+    The reference forms the full DM and traces out the other modes.  For a ket
+    the same sum, rho_red = sum over traced indices of psi psi*, is contracted
+    directly so the D^(2n) array is never built; mixed states take the
+    reference's route."""
     traced = [m for m in range(n) if m not in modes]
+    if pure:
+        a = list(_IDX[:n])
+        b = [a[m] if m in traced else _IDX[n + m] for m in range(n)]
+        out = "".join(a[m] + b[m] for m in sorted(modes))
+        return np.einsum("".join(a) + "," + "".join(b) + "->" + out, state, state.conj())
+    rho = dm_of(state, n, pure)
     return partial_trace(rho, n, traced)
 
 

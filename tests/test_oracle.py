@@ -325,3 +325,16 @@ def test_coherent_mean_photon():
     st, pure = fo.run_ops([("D", (0,), (r, 0.1))], 1, D)
     m, v = fo.mean_photon(st, 1, pure, 0)
     assert abs(m - r**2) < 1e-10 and abs(v - r**2) < 1e-9
+
+
+def test_reduced_dm_of_ket_equals_partial_trace_of_full_dm():
+    """The ket route of reduced_dm contracts psi psi* over the traced modes;
+    it must equal the reference route (full DM, then partial trace)."""
+    rng = np.random.default_rng(7)
+    for n, D in [(2, 5), (3, 4), (4, 3)]:
+        psi = rng.normal(size=[D] * n) + 1j * rng.normal(size=[D] * n)
+        for k in range(1, n + 1):
+            for modes in itertools.combinations(range(n), k):
+                traced = [m for m in range(n) if m not in modes]
+                full = fo.partial_trace(fo.dm_of(psi, n, True), n, traced)
+                assert np.abs(fo.reduced_dm(psi, n, True, list(modes)) - full).max() < 1e-12
