@@ -46,6 +46,12 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline sample")
     ap.add_argument("--fuse", type=int, default=None, help="local modes per fused stage (0 = one kernel per gate)")
     ap.add_argument(
+        "--kernels-only",
+        action="store_true",
+        help="c3: run only the captured evaluator and the event-timed sweeps (no eager, TF-path or CPU legs); "
+        "the short run that hardware-counter profiles use",
+    )
+    ap.add_argument(
         "--workload",
         default="c3",
         choices=["c2", "c3", "c4", "c5", "hbm"],
@@ -92,6 +98,10 @@ def kernel_rates(n, D, mixed, dtype, reps=10):
     if mixed:
         kinds += [("LOSS", (0,)), ("LOSS", (n - 1,))]
     res = {}
+    # plain device copy of the same state: the bandwidth reference (and the
+    # kernel whose HBM counters calibrate FETCH_SIZE / WRITE_SIZE)
+    tc = _event_us(lambda: out.copy_(st), reps)
+    res["copy"] = {"fwd_us": round(tc, 1), "fwd_GBs": round(2 * S / tc / 1e3, 1)}
     for kind, modes in kinds:
         cc = runtime.CompiledCircuit([(kind, modes)], n, D, mixed, dtype, dev, fuse=0)
         cc.build_table(torch.tensor([[0.3, 0.2] if kind != "LOSS" else [0.8, 0.0]], dtype=torch.float64, device=dev))
@@ -101,8 +111,10 @@ def kernel_rates(n, D, mixed, dtype, reps=10):
         res["%s%s" % (kind, list(modes))] = {
             "fwd_us": round(tf, 1),
             "fwd_GBs": round(2 * S / tf / 1e3, 1),
+            "fwd_pct_peak": round(100 * 2 * S / tf / 1e3 / HBM_PEAK_GBS, 1),
             "adj_us": round(ta, 1),
             "adj_GBs": round(4 * S / ta / 1e3, 1),
+            "adj_pct_peak": round(100 * 4 * S / ta / 1e3 / HBM_PEAK_GBS, 1),
         }
     return S, res
 
@@ -228,7 +240,7 @@ def run_c2(args):
             "fock_prob_1021": float(probs[1, 0, 2, 1]),
         }
     cpu = None
-    if not args.no_cpu:
+    if not (args.no_cpu or args.kernels_only):
         from threadpoolctl import threadpool_limits
 
         from oracle import fock_oracle as fo
@@ -326,6 +338,45 @@ def run_hbm(args):
     )
 
 
+def comparators(args, cc, params, vac, target, grads, ops, n, D, timed):
+    """Eager autograd (same kernels, no graph) and the TF-backend gradient path
+    restated in torch (dense einsums + autograd) on the same GPU."""
+    import torch
+
+    from strawberryfields_amd import runtime
+
+    dtype = vac.dtype
+    dev = vac.device
+    p_leaf = params.clone().requires_grad_(True)
+
+    def eager():
+        p_leaf.grad = None
+        psi = runtime.CircuitFunction.apply(p_leaf, vac, cc, None)
+        runtime.fidelity_loss(psi, target).backward()
+
+    t_eager = timed(eager, max(10, args.steps // 4), 3)
+    eager_rate = max(10, args.steps // 4) / t_eager
+    eager_diff = float((p_leaf.grad - grads).abs().max())
+
+    from strawberryfields_amd import tfpath
+
+    tfc = tfpath.TFPathCircuit([(o[0], o[1]) for o in ops], n, D, dtype, dev)
+    p_tf = params.clone().requires_grad_(True)
+
+    def tf_step():
+        p_tf.grad = None
+        psi = tfc.forward(p_tf)
+        ov = torch.vdot(target.reshape(-1), psi.reshape(-1))
+        (1 - ov.real**2 - ov.imag**2).backward()
+
+    tf_steps = 5
+    t_tf = timed(tf_step, tf_steps, 1)
+    tf_rate = tf_steps / t_tf
+    tf_diff = float((p_tf.grad - grads).abs().max() / grads.abs().max())
+
+    return eager_rate, eager_diff, tf_rate, tf_diff
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -388,34 +439,9 @@ def main():
     t_fwd = timed(lambda: evf(params), args.steps, args.warmup)
     fwd_rate = args.steps / t_fwd
 
-    # ---- eager autograd path (no graph), same kernels
-    p_leaf = params.clone().requires_grad_(True)
-
-    def eager():
-        p_leaf.grad = None
-        psi = runtime.CircuitFunction.apply(p_leaf, vac, cc, None)
-        runtime.fidelity_loss(psi, target).backward()
-
-    t_eager = timed(eager, max(10, args.steps // 4), 3)
-    eager_rate = max(10, args.steps // 4) / t_eager
-    eager_diff = float((p_leaf.grad - grads).abs().max())
-
-    # ---- TF-backend gradient path restated in torch (dense einsums + autograd)
-    from strawberryfields_amd import tfpath
-
-    tfc = tfpath.TFPathCircuit([(o[0], o[1]) for o in ops], n, D, dtype, dev)
-    p_tf = params.clone().requires_grad_(True)
-
-    def tf_step():
-        p_tf.grad = None
-        psi = tfc.forward(p_tf)
-        ov = torch.vdot(target.reshape(-1), psi.reshape(-1))
-        (1 - ov.real**2 - ov.imag**2).backward()
-
-    tf_steps = 5
-    t_tf = timed(tf_step, tf_steps, 1)
-    tf_rate = tf_steps / t_tf
-    tf_diff = float((p_tf.grad - grads).abs().max() / grads.abs().max())
+    eager_rate = eager_diff = tf_rate = tf_diff = None
+    if not args.kernels_only:
+        eager_rate, eager_diff, tf_rate, tf_diff = comparators(args, cc, params, vac, target, grads, ops, n, D, timed)
 
     # ---- sweep timing with HIP events on the evaluator's stream
     S = (D**n) * 8
@@ -460,7 +486,7 @@ def main():
 
     # ---- CPU baseline: faithful fock-backend restatement, one forward pass
     cpu = None
-    if not args.no_cpu:
+    if not (args.no_cpu or args.kernels_only):
         from threadpoolctl import threadpool_limits
 
         from oracle import fock_oracle as fo
@@ -504,9 +530,9 @@ def main():
             "steps_per_sweep": cc.nsteps,
         },
         "fwd_evals_per_s": round(fwd_rate, 2),
-        "eager_grad_evals_per_s": round(eager_rate, 2),
-        "tf_path_grad_evals_per_s": round(tf_rate, 3),
-        "speedup_vs_tf_path": round(value / tf_rate, 1),
+        "eager_grad_evals_per_s": round(eager_rate, 2) if eager_rate else None,
+        "tf_path_grad_evals_per_s": round(tf_rate, 3) if tf_rate else None,
+        "speedup_vs_tf_path": round(value / tf_rate, 1) if tf_rate else None,
         "tf_path_grad_rel_diff": tf_diff,
         "graph_vs_eager_max_grad_diff": eager_diff,
         "loss": loss,

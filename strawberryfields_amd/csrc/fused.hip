@@ -279,6 +279,24 @@ __device__ __forceinline__ int64_t tile_base(int64_t t, const sfamd::StageArgs& 
   return base;
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8),
+// and each XCD has its own L2.  Neighbouring tiles share cache lines whenever
+// the innermost non-local mode has a small stride, so the tiles are cut into 8
+// contiguous ranges and XCD x's blocks (b = x + 8 k) walk range x; every line
+// is then fetched into one L2 instead of eight.
+struct TileRange {
+  int64_t first, end, step;
+};
+
+__device__ __forceinline__ TileRange xcd_tiles(int64_t ntiles) {
+  const int64_t G = gridDim.x, b = blockIdx.x;
+  if (G % 8) return TileRange{b, ntiles, G};
+  const int64_t per = (ntiles + 7) / 8;
+  const int64_t x = b % 8;
+  const int64_t end = (x + 1) * per < ntiles ? (x + 1) * per : ntiles;
+  return TileRange{x * per + b / 8, end, G / 8};
+}
+
 // ---- kernels ------------------------------------------------------------
 template <typename T, int D, int K, int NT = sfamd::stage_threads(Pow<D, K>::v)>
 __global__ void __launch_bounds__(NT) k_stage_fwd(const C<T>* __restrict__ in, C<T>* __restrict__ out,
@@ -289,7 +307,8 @@ __global__ void __launch_bounds__(NT) k_stage_fwd(const C<T>* __restrict__ in, C
   C<T>* bufB = bufA + DK;
   C<T>* mats = bufB + DK;
   stage_all_matrices<T, D, NT>(table, a, false, mats);
-  for (int64_t t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
+  const TileRange tr = xcd_tiles(a.ntiles);
+  for (int64_t t = tr.first; t < tr.end; t += tr.step) {
     const int64_t base = tile_base(t, a, D);
     for (int e = threadIdx.x; e < DK; e += NT) bufA[e] = in[base + local_offset<D, K>(e, a)];
     __syncthreads();
@@ -323,7 +342,8 @@ __global__ void __launch_bounds__(NT) k_stage_adj(const C<T>* __restrict__ psi_i
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (int idx = threadIdx.x; idx < a.ngates * 2 * kWaves; idx += NT) gsl[idx] = 0.0;
   stage_all_matrices<T, D, NT>(table, a, true, mats);
-  for (int64_t t = blockIdx.x; t < a.ntiles; t += gridDim.x) {
+  const TileRange tr = xcd_tiles(a.ntiles);
+  for (int64_t t = tr.first; t < tr.end; t += tr.step) {
     const int64_t base = tile_base(t, a, D);
     for (int e = threadIdx.x; e < DK; e += NT) {
       const int64_t off = base + local_offset<D, K>(e, a);
