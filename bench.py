@@ -338,6 +338,20 @@ def run_hbm(args):
     )
 
 
+def stage_adj_traffic():
+    """HBM bytes per k_stage_adj launch measured by the FETCH_SIZE / WRITE_SIZE
+    counter runs recorded in profiles/ (calibrated against a device copy)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "c3_stage_adj_hbm_traffic.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch k_stage_adj:"):
+                    return int(line.split(":")[1]), "profiles/c3_stage_adj_hbm_traffic.txt"
+    except OSError:
+        pass
+    return None, None
+
+
 def comparators(args, cc, params, vac, target, grads, ops, n, D, timed):
     """Eager autograd (same kernels, no graph) and the TF-backend gradient path
     restated in torch (dense einsums + autograd) on the same GPU."""
@@ -465,6 +479,7 @@ def main():
     t_fsweep = event_time(lambda: cc.forward(0, cc.nsteps, ev.state_in, out=ev.psi, saved=ev.saved), reps)
     t_asweep = event_time(lambda: cc.adjoint(0, cc.nsteps, ev.saved, ev.g_psi, ev.g_in, ev.grads), reps)
     nops = len(ops)
+    traffic, traffic_src = stage_adj_traffic()
     adj_bytes = 4 * S * nops
     fwd_bytes = 2 * S * nops
     achieved = adj_bytes / t_asweep / 1e9
@@ -475,7 +490,8 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": traffic,
+        "traffic_source": traffic_src,
         "bytes_per_launch": int(adj_bytes / cc.nsteps),
         "avg_launch_us": round(t_asweep / cc.nsteps * 1e6, 3),
         "algorithmic": "4 S per adjoint operation, S = %d B; %d operations in %d fused launches" % (S, nops, cc.nsteps),

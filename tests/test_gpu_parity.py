@@ -636,3 +636,67 @@ def test_slab_kernels_match_oracle_and_fd(dtype, pure, n, D, pairs):
     g_fd = _fd_subset(ops, n, D, pure, weights, sel)
     rel = np.linalg.norm(gflat[sel] - g_fd) / np.linalg.norm(g_fd)
     assert rel < (1e-7 if dtype == C128 else 1e-4), rel
+
+
+# ------------------------------------------------------------------ c5 structure at a smaller cutoff
+_C5S = {}
+
+
+def _c5_structure_reference():
+    """Oracle forward and central differences (4 seeded parameters) of the
+    c5-shaped circuit at cutoff 10, computed once for both dtypes."""
+    if _C5S:
+        return _C5S
+    n, D, L = 3, 10, 2
+    rng = np.random.default_rng(20261205)
+    ops = circuits.layered(n, L, rng, sd=0.1, loss=True)
+    vac = fo.vacuum(n, D, pure=False)
+    ref, _ = fo.run_ops(ops, n, D, pure=False, state=vac)
+    idx = np.arange(D, dtype=float)
+    w = (idx[:, None, None] + idx[None, :, None] + idx[None, None, :] - 3.0) ** 2
+    flat = np.array([v for _, _, p in ops for v in p], dtype=float)
+    sel = np.sort(rng.choice(len(flat), size=4, replace=False))
+
+    def loss_at(vals):
+        k, new = 0, []
+        for name, modes, p in ops:
+            new.append((name, modes, tuple(vals[k : k + len(p)])))
+            k += len(p)
+        st, _ = fo.run_ops(new, n, D, pure=False, state=vac)
+        return float((fo.all_fock_probs(st, n, False) * w).sum())
+
+    h = 1e-5
+    g_fd = []
+    for i in sel:
+        a, b = flat.copy(), flat.copy()
+        a[i] += h
+        b[i] -= h
+        g_fd.append((loss_at(a) - loss_at(b)) / (2 * h))
+    _C5S.update(n=n, D=D, L=L, ops=ops, vac=vac, ref=ref, w=w, sel=sel, g_fd=np.array(g_fd))
+    return _C5S
+
+
+@pytest.mark.parametrize("dtype", [C128, C64])
+def test_c5_structure_dm_layers_with_loss_checkpoints(dtype):
+    """The c5 circuit shape (3-mode DM from vacuum, layered circuit with a loss
+    channel per mode, loss sum_n p(n) (n1+n2+n3-3)^2, layer-boundary checkpoints)
+    at cutoff 10 (10^6 DM entries), where every c5 kernel route is taken
+    (one-pass two-sided dense gates on inner strides 1 and >= 16, beamsplitters
+    on strided and slab tiles, slab loss) and the complex128 oracle runs in
+    seconds: forward DM vs the oracle, gradient vs central differences."""
+    r = _c5_structure_reference()
+    n, D, ops = r["n"], r["D"], r["ops"]
+    cc = runtime.CompiledCircuit([(o[0], o[1]) for o in ops], n, D, True, dtype, DEV)
+    rows = np.zeros((len(ops), 2))
+    for i, (_, _, p) in enumerate(ops):
+        rows[i, : len(p)] = p
+    params = torch.tensor(rows, dtype=torch.float64, device=DEV, requires_grad=True)
+    rho0 = torch.as_tensor(r["vac"], dtype=dtype, device=DEV)
+    rho = runtime.CircuitFunction.apply(params, rho0, cc, len(ops) // r["L"])
+    assert np.abs(to_np(rho) - r["ref"]).max() < TOL[dtype] * 10
+    probs = runtime.all_fock_probs(rho, n, D, True).to(torch.float64)
+    (probs * torch.as_tensor(r["w"], device=DEV)).sum().backward()
+    g = params.grad.cpu().numpy()
+    gflat = np.array([g[i, j] for i, (_, _, p) in enumerate(ops) for j in range(len(p))])
+    rel = np.linalg.norm(gflat[r["sel"]] - r["g_fd"]) / np.linalg.norm(r["g_fd"])
+    assert rel < (1e-7 if dtype == C128 else 1e-4), rel
