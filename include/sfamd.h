@@ -48,7 +48,12 @@ enum {
   SFAMD_OP_SQUEEZE = 3,      /* S(r, phi)     fockbackend/ops.py:238-254       */
   SFAMD_OP_BEAMSPLITTER = 4, /* BS(theta,phi) fockbackend/ops.py:318-329       */
   SFAMD_OP_LOSS = 5,         /* N(T)          fockbackend/ops.py:471-490       */
-  SFAMD_OP_MIX = 6           /* pure -> mixed fockbackend/ops.py:110-120       */
+  SFAMD_OP_MIX = 6,          /* pure -> mixed fockbackend/ops.py:110-120       */
+  SFAMD_OP_TWO_MODE_SQUEEZE = 7, /* S2(r, phi)  fockbackend/ops.py:257-271,
+                                    circuit.py:572-577 (_apply_S2 337-365)      */
+  SFAMD_OP_MZGATE = 8,       /* MZ(phi_in, phi_ex) fockbackend/ops.py:332-343  */
+  SFAMD_OP_CROSS_KERR = 9,   /* CK(kappa)     fockbackend/ops.py:284-294       */
+  SFAMD_OP_CUBIC_PHASE = 10  /* V(gamma), hbar = 2  fockbackend/ops.py:297-306 */
 };
 
 /* error codes (negative) */
@@ -62,7 +67,8 @@ enum {
 
 /* maximum cutoff accepted by each gate family */
 #define SFAMD_MAX_CUTOFF 64      /* R, K, D, S, loss                          */
-#define SFAMD_MAX_CUTOFF_BS 40   /* beamsplitter (two FP64 blocks in LDS)     */
+#define SFAMD_MAX_CUTOFF_BS 40   /* beamsplitter, S2, MZ, CK (two FP64 blocks in LDS) */
+#define SFAMD_MAX_CUTOFF_V 32    /* cubic phase (FP64 matrix exponential in LDS) */
 
 /* ---- library ------------------------------------------------------------ */
 int sfamd_version(void);
@@ -86,6 +92,14 @@ int64_t sfamd_gate_table_size(int kind, int D);
  *          followed by the same blocks of dB/dtheta.
  *   loss : real c[p][q][l] then real dc/dT[p][q][l], packed two reals per
  *          complex slot; rho'[p,q] = sum_l c[p][q][l] rho[p+l][q+l].
+ *   MZ, CK: photon-number blocks as BS (both conserve N1 + N2), then the same
+ *          blocks of dG/dp0 (p0 = phi_in, kappa).  The second parameter's
+ *          derivative is i w G[N][n] with w = n (MZ: phi_ex rotates input
+ *          mode 1) and none for CK; for BS it is w = n - N.
+ *   S2   : blocks of constant out1 - out2 = in1 - in2 = d, stored as BS block
+ *          s = D - 1 + d holding G[N][n] (out1 = N, in1 = n, out2 = N - d,
+ *          in2 = n - d), then the blocks of dG/dr; dG/dphi = i (N - n) G.
+ *   V    : U[D][D], dU/dgamma[D][D], zeros[D][D] (as D, S).
  */
 int sfamd_build_gate(int kind, double p0, double p1, int D, int dtype,
                      void* out, void* stream);
@@ -93,7 +107,7 @@ int sfamd_build_gate(int kind, double p0, double p1, int D, int dtype,
 /* ---- single operations on a state --------------------------------------- */
 /* Dense or diagonal single-mode gate.  Replaces Circuit.apply_gate_BLAS
  * (fockbackend/circuit.py:118-217) for one mode.  `table` is the gate-table
- * entry of a R/K/D/S gate.  For a DM (mixed=1) the gate is applied on both
+ * entry of a R/K/D/S/V gate.  For a DM (mixed=1) the gate is applied on both
  * sides, U rho U^dagger.  `tmp` (same size as the state) is needed for a dense
  * gate on a DM and may be NULL otherwise.  in == out is not allowed. */
 int sfamd_apply_1mode(int kind, const void* in, void* out, void* tmp,
@@ -106,6 +120,14 @@ int sfamd_apply_1mode(int kind, const void* in, void* out, void* tmp,
 int sfamd_apply_bs(const void* in, void* out, void* tmp, int n, int D,
                    int mode1, int mode2, int mixed, const void* table,
                    int dtype, void* stream);
+
+/* Any two-mode gate (kind BS, S2, MZ or CK) on (mode1, mode2), the table's
+ * first pair on mode1.  Replaces Circuit.apply_twomode_gate with the
+ * BSgate / MZgate / S2gate branches (fockbackend/circuit.py:219-304) and the
+ * cross-Kerr apply_gate_BLAS call (circuit.py:586-591). */
+int sfamd_apply_2mode(int kind, const void* in, void* out, void* tmp, int n,
+                      int D, int mode1, int mode2, int mixed,
+                      const void* table, int dtype, void* stream);
 
 /* Loss channel on a DM.  Replaces Circuit._apply_channel with the Kraus set of
  * fockbackend/ops.py:471-490 (fockbackend/circuit.py:65-87). */

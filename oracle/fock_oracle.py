@@ -12,10 +12,14 @@ What it restates (reference files, read as text, never imported):
   ``thewalrus.fock_gradients`` 0.22.0, not vendored; its published algorithm is
   a ladder-operator recurrence, restated below from the Heisenberg action of
   each gate), ``lossChannel`` Kraus set 471-490, ``a`` 208-216, ``proj`` 346-353;
-  the loss superoperator of ``tfbackend/ops.py:288-315``.
+  the loss superoperator of ``tfbackend/ops.py:288-315``; ``two_mode_squeeze``
+  257-271 and ``mzgate`` 332-343 (thewalrus, not vendored: S2 by its Gaussian
+  Hermite recurrence, MZ as the product of its documented factors),
+  ``cross_kerr`` 284-294, ``cubicPhase`` 296-306.
 * state updates -- ``strawberryfields/backends/fockbackend/circuit.py``:
   ``apply_gate_BLAS`` 118-217 (the slice loop with ``np.dot``), the
-  selection-rule beamsplitter loop ``_apply_two_mode_passive`` 306-335,
+  selection-rule beamsplitter loop ``_apply_two_mode_passive`` 306-335, the
+  difference-conserving ``_apply_S2`` 337-365,
   ``_apply_channel`` 65-87, ``prepare_multimode`` 393-473, ``norm`` 367-371,
   ``is_vacuum`` 600-609; ``ops.mix`` 110-120, ``partial_trace`` 144-157.
 * readout -- ``strawberryfields/backends/states.py`` ``BaseFockState``:
@@ -29,12 +33,16 @@ this oracle (like the TF backend, ``tfbackend/ops.py:800-811``, and the
 ``BaseBackend.beamsplitter`` contract) applies the matrix's first mode pair to
 ``mode1``.  For ``mode1 < mode2`` both agree.
 
-Pinning (see tests/test_oracle.py): the D, S, BS matrices are checked against
-50-digit mpmath closed forms (tests/golden/gates_mpmath.npz), the BS
-combinatorial prefactors against the reference's own
-``fock_beamsplitter_factors_{5,10}.npz``, the Kraus loss against the
-superoperator identity, the slice-loop gate application against a brute-force
-Kronecker operator, and the restated reference known-answer tests.
+Pinning (see tests/test_oracle.py): the D, S, BS and S2 matrices are checked
+against 50-digit mpmath closed forms (tests/golden/gates_mpmath.npz), S2 also
+against expm of its generator on a 48-level basis, the BS combinatorial
+prefactors against the reference's own ``fock_beamsplitter_factors_{5,10}.npz``,
+MZ against its documented mode unitary, special cases and gate decomposition,
+V against the eigendecomposition of x^3, the Kraus loss against the
+superoperator identity, the slice-loop and selection-rule gate applications
+against a brute-force Kronecker operator, and the restated reference
+known-answer tests (TMSV amplitudes, MZ on |1,1>, cross-Kerr and cubic phase on
+uniform states, coherent and squeezed states, Hong-Ou-Mandel, loss).
 """
 from itertools import product
 import string
@@ -183,6 +191,91 @@ def loss_superop(T, D):
     return S
 
 
+def two_mode_squeezing(r, phi, D):
+    """S2(r, phi) as mat[out1, in1, out2, in2] (fockbackend/ops.py:257-271 layout).
+
+    S2 = exp(r (e^{i phi} a1^dag a2^dag - e^{-i phi} a1 a2)), the convention the
+    reference's known-answer test fixes (test_twomode_squeezing_operation.py:30-42:
+    S2|0,0> = sech r sum_k (e^{i phi} tanh r)^k |k,k>).  thewalrus 0.22.0
+    ``two_mode_squeezing`` is not vendored; its published method is a
+    Gaussian-kernel (Hermite) recurrence in which every index step down-shifts
+    the others, restated here from the Heisenberg action
+    S2^dag a1 S2 = cosh r a1 + e^{i phi} sinh r a2^dag (and 1 <-> 2):
+      sqrt(N) G[N, M, n, m] = sech r sqrt(n) G[N-1, M, n-1, m] + e^{i phi} tanh r sqrt(M) G[N-1, M-1, n, m]
+      sqrt(M) G[N, M, n, m] = sech r sqrt(m) G[N, M-1, n, m-1] + e^{i phi} tanh r sqrt(N) G[N-1, M-1, n, m]
+      sqrt(n) G[N, M, n, m] = sech r sqrt(N) G[N-1, M, n-1, m] - e^{-i phi} tanh r sqrt(m) G[N, M, n-1, m-1]
+    with G[N, M, n, m] = <N,M|S2|n,m>, G[0,0,0,0] = sech r, and entries zero
+    unless N - M = n - m.  Every coefficient is at most 1 in magnitude and no
+    entry outside the D^4 block is read, so the truncation is exact.
+    """
+    sech, th = 1.0 / np.cosh(r), np.tanh(r)
+    e = np.exp(1j * phi)
+    sq = np.sqrt(np.arange(D + 1, dtype=float))
+    G = np.zeros((D, D, D, D), dtype=DEF_TYPE)
+    G[0, 0, 0, 0] = sech
+    for N in range(D):
+        for M in range(D):
+            for n in range(D):
+                m = n - N + M
+                if m < 0 or m >= D or N + M + n + m == 0:
+                    continue
+                if N > 0:
+                    v = (sech * sq[n] * G[N - 1, M, n - 1, m] if n > 0 else 0.0) + (
+                        e * th * sq[M] * G[N - 1, M - 1, n, m] if M > 0 else 0.0
+                    )
+                    v /= sq[N]
+                elif M > 0:  # N = 0
+                    v = (sech * sq[m] * G[0, M - 1, n, m - 1] if m > 0 else 0.0) / sq[M]
+                else:  # N = M = 0, so n = m > 0
+                    v = -np.conj(e) * th * sq[m] * G[0, 0, n - 1, m - 1] / sq[n]
+                G[N, M, n, m] = v
+    return G.transpose(0, 2, 1, 3).copy()
+
+
+_BS50 = {}  # BS(pi/4, pi/2) at cutoff L as an (L^2, L^2) operator, by L
+
+
+def mzgate(phi_in, phi_ex, D):
+    """MZ(phi_in, phi_ex) as mat[out1, in1, out2, in2] (fockbackend/ops.py:332-343 layout).
+
+    The operator of strawberryfields/ops.py:1951-1990,
+    MZ = BS(pi/4, pi/2) (R(phi_in) x I) BS(pi/4, pi/2) (R(phi_ex) x I), whose mode
+    unitary is 1/2 [[(e^{i phi_in} - 1) e^{i phi_ex}, i (1 + e^{i phi_in})],
+    [i (1 + e^{i phi_in}) e^{i phi_ex}, 1 - e^{i phi_in}]] (thewalrus 0.22.0
+    ``mzgate`` is not vendored).  The four factors are multiplied as Fock
+    operators at cutoff 2D - 1: every factor conserves n1 + n2 <= 2D - 2 on the
+    kept block, so the product is exact there before it is truncated to D.
+    """
+    L = max(2 * D - 1, 1)
+    if L not in _BS50:
+        _BS50[L] = beamsplitter(np.pi / 4, np.pi / 2, L).transpose(0, 2, 1, 3).reshape(L * L, L * L)
+    B = _BS50[L]
+    ones = np.ones(L)
+    Rin = np.kron(np.exp(1j * phi_in * np.arange(L)), ones)
+    Rex = np.kron(np.exp(1j * phi_ex * np.arange(L)), ones)
+    M = (B * Rin[None, :]) @ (B * Rex[None, :])
+    M = M.reshape(L, L, L, L)[:D, :D, :D, :D]  # [out1, out2, in1, in2]
+    return M.transpose(0, 2, 1, 3).copy()
+
+
+def cross_kerr(kappa, D):
+    """CK(kappa) = exp(i kappa n1 n2) as mat[out1, in1, out2, in2] (fockbackend/ops.py:284-294)."""
+    n = np.arange(D)
+    mat = np.zeros((D, D, D, D), dtype=DEF_TYPE)
+    mat[n[:, None], n[:, None], n[None, :], n[None, :]] = np.exp(1j * kappa * np.outer(n, n))
+    return mat
+
+
+def cubic_phase(gamma, D, hbar=2.0):
+    """V(gamma) = expm(i gamma / (3 hbar) x^3), x = sqrt(hbar/2) (a + a^dag) on the
+    truncated basis (fockbackend/ops.py:297-306; scipy.linalg.expm as there)."""
+    from scipy.linalg import expm
+
+    a = annihilation(D)
+    x = (a + a.conj().T) * np.sqrt(hbar / 2)
+    return expm(1j * gamma / (3 * hbar) * (x @ x @ x)).astype(DEF_TYPE)
+
+
 # ----------------------------------------------------------------- states
 def mix(state, n):
     """rho[i0,j0,i1,j1,...] = psi[i...] psi*[j...]  (fockbackend/ops.py:110-120)."""
@@ -311,21 +404,47 @@ class Circuit:
             ret[ks, s - ks] = np.tensordot(blk, sub, axes=(1, 0))
         return ret
 
-    def apply_twomode_gate(self, mat, modes):
-        D, n = self._trunc, self._num_modes
+    # -- two-mode squeezing (circuit.py:337-365), first matrix pair on modes[0]
+    @staticmethod
+    def _two_mode_diffcons(mat, state, D, faithful):
+        """ret[i, k, ...] = sum_j mat[i, j, k, k+j-i] state[j, k+j-i, ...]  (circuit.py:337-365)."""
+        ret = np.zeros_like(state, dtype=DEF_TYPE)
+        if faithful:
+            for i in range(D):
+                for j in range(D):
+                    for k in range(max(i - j, 0), D + min(i - j, 0)):
+                        ret[i, k] += mat[i, j, k, k + j - i] * state[j, k + j - i]
+            return ret
+        for d in range(-(D - 1), D):
+            # out1 - out2 = in1 - in2 = d
+            ks = np.arange(max(0, d), min(D, D + d))
+            blk = mat[ks[:, None], ks[None, :], (ks - d)[:, None], (ks - d)[None, :]]
+            ret[ks, ks - d] = np.tensordot(blk, state[ks, ks - d], axes=(1, 0))
+        return ret
+
+    def apply_twomode_gate(self, mat, modes, gate="BSgate"):
+        """circuit.py:219-304: BSgate / MZgate by the passive selection rule,
+        S2gate by the difference-conserving one; bra side with mat, ket side
+        with conj(mat)."""
+        D = self._trunc
         st = self._state
+        rule = {"BSgate": self._two_mode_passive, "MZgate": self._two_mode_passive}.get(gate)
+        if gate == "S2gate":
+            rule = self._two_mode_diffcons
+        if rule is None:
+            raise NotImplementedError(gate)
         if self._pure:
             axes = [modes[0], modes[1]]
             v = np.moveaxis(st, axes, [0, 1])
-            v = self._two_mode_passive(mat, v, D, self.faithful)
+            v = rule(mat, v, D, self.faithful)
             return np.moveaxis(v, [0, 1], axes)
         bra = [2 * modes[0], 2 * modes[1]]
         ket = [2 * modes[0] + 1, 2 * modes[1] + 1]
         v = np.moveaxis(st, bra, [0, 1])
-        v = self._two_mode_passive(mat, v, D, self.faithful)
+        v = rule(mat, v, D, self.faithful)
         v = np.moveaxis(v, [0, 1], bra)
         v = np.moveaxis(v, ket, [0, 1])
-        v = self._two_mode_passive(mat.conj(), v, D, self.faithful)
+        v = rule(mat.conj(), v, D, self.faithful)
         return np.moveaxis(v, [0, 1], ket)
 
     # -- channels (circuit.py:65-87)
@@ -354,6 +473,18 @@ class Circuit:
 
     def loss(self, T, mode):
         self._apply_channel(loss_kraus(T, self._trunc), [mode])
+
+    def mzgate(self, phi_in, phi_ex, mode1, mode2):
+        self._state = self.apply_twomode_gate(mzgate(phi_in, phi_ex, self._trunc), [mode1, mode2], "MZgate")
+
+    def two_mode_squeeze(self, r, phi, mode1, mode2):
+        self._state = self.apply_twomode_gate(two_mode_squeezing(r, phi, self._trunc), [mode1, mode2], "S2gate")
+
+    def cross_kerr_interaction(self, kappa, mode1, mode2):
+        self._state = self.apply_gate_BLAS(cross_kerr(kappa, self._trunc), [mode1, mode2])
+
+    def cubic_phase_shift(self, gamma, mode):
+        self._state = self.apply_gate_BLAS(cubic_phase(gamma, self._trunc), [mode])
 
     # -- preparations (circuit.py:393-498), single mode
     def prepare(self, st, mode):
@@ -461,7 +592,8 @@ def mean_photon(state, n, pure, mode):
 def run_ops(ops, n, D, pure=True, faithful=False, state=None):
     """Apply a list of (name, modes, params) ops to vacuum (or `state`).
 
-    name in {"R", "K", "D", "S", "BS", "LOSS"}; returns (state, pure).
+    name in {"R", "K", "D", "S", "BS", "LOSS", "S2", "MZ", "CK", "V"} (V takes the
+    hbar = 2 parameter of the backend call); returns (state, pure).
     """
     c = Circuit(n, D, pure=pure, faithful=faithful)
     if state is not None:
@@ -480,6 +612,14 @@ def run_ops(ops, n, D, pure=True, faithful=False, state=None):
             c.beamsplitter(p[0], p[1], modes[0], modes[1])
         elif name == "LOSS":
             c.loss(p[0], modes[0])
+        elif name == "S2":
+            c.two_mode_squeeze(p[0], p[1], modes[0], modes[1])
+        elif name == "MZ":
+            c.mzgate(p[0], p[1], modes[0], modes[1])
+        elif name == "CK":
+            c.cross_kerr_interaction(p[0], modes[0], modes[1])
+        elif name == "V":
+            c.cubic_phase_shift(p[0], modes[0])
         else:
             raise ValueError(name)
     return c.get_state()

@@ -12,7 +12,19 @@ import torch  # noqa: F401  (loads the HIP runtime the library binds to)
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libsfamd.so")
 
 C64, C128 = 0, 1
-OP_ROTATION, OP_KERR, OP_DISPLACEMENT, OP_SQUEEZE, OP_BEAMSPLITTER, OP_LOSS, OP_MIX = range(7)
+(
+    OP_ROTATION,
+    OP_KERR,
+    OP_DISPLACEMENT,
+    OP_SQUEEZE,
+    OP_BEAMSPLITTER,
+    OP_LOSS,
+    OP_MIX,
+    OP_TWO_MODE_SQUEEZE,
+    OP_MZGATE,
+    OP_CROSS_KERR,
+    OP_CUBIC_PHASE,
+) = range(11)
 OP_KINDS = {
     "R": OP_ROTATION,
     "K": OP_KERR,
@@ -21,9 +33,15 @@ OP_KINDS = {
     "BS": OP_BEAMSPLITTER,
     "LOSS": OP_LOSS,
     "MIX": OP_MIX,
+    "S2": OP_TWO_MODE_SQUEEZE,
+    "MZ": OP_MZGATE,
+    "CK": OP_CROSS_KERR,
+    "V": OP_CUBIC_PHASE,
 }
+TWO_MODE = ("BS", "S2", "MZ", "CK")
 MAX_CUTOFF = 64
-MAX_CUTOFF_BS = 40
+MAX_CUTOFF_BS = 40  # beamsplitter, S2, MZ, CK
+MAX_CUTOFF_V = 32  # cubic phase
 
 
 class SfamdError(RuntimeError):
@@ -65,6 +83,7 @@ SIGNATURES = {
     "sfamd_build_gate": (_I, [_I, _D, _D, _I, _I, _P, _P]),
     "sfamd_apply_1mode": (_I, [_I, _P, _P, _P, _I, _I, _I, _I, _P, _I, _P]),
     "sfamd_apply_bs": (_I, [_P, _P, _P, _I, _I, _I, _I, _I, _P, _I, _P]),
+    "sfamd_apply_2mode": (_I, [_I, _P, _P, _P, _I, _I, _I, _I, _I, _P, _I, _P]),
     "sfamd_apply_loss": (_I, [_P, _P, _I, _I, _I, _P, _I, _P]),
     "sfamd_mix": (_I, [_P, _P, _I, _I, _I, _P]),
     "sfamd_replace_mode": (_I, [_P, _P, _I, _I, _I, _P, _I, _P]),

@@ -170,6 +170,31 @@ class HIPFockBackend(BaseFock):
     def loss(self, T, mode):
         self._record("LOSS", [self._remap_modes(mode)], [T])
 
+    def _two_mode(self, name, modes, params):
+        if self._D > _lib.MAX_CUTOFF_BS:
+            raise ValueError("two-mode gates support cutoff_dim <= %d" % _lib.MAX_CUTOFF_BS)
+        self._record(name, [self._remap_modes(m) for m in modes], params)
+
+    def two_mode_squeeze(self, r, phi, mode1, mode2):
+        """S2(r e^{i phi}) = exp(r (e^{i phi} a1^dag a2^dag - e^{-i phi} a1 a2))
+        (fockbackend/backend.py:175-176)."""
+        self._two_mode("S2", [mode1, mode2], [r, phi])
+
+    def mzgate(self, phi_in, phi_ex, mode1, mode2):
+        """Mach-Zehnder interferometer (fockbackend/backend.py:181-182)."""
+        self._two_mode("MZ", [mode1, mode2], [phi_in, phi_ex])
+
+    def cross_kerr_interaction(self, kappa, mode1, mode2):
+        """exp(i kappa n1 n2) (fockbackend/backend.py:285-288)."""
+        self._two_mode("CK", [mode1, mode2], [kappa])
+
+    def cubic_phase(self, gamma, mode):
+        """expm(i gamma/6 (a + a^dag)^3) on the truncated basis, the hbar = 2
+        matrix of fockbackend/ops.py:297-306 (fockbackend/backend.py:279-280)."""
+        if self._D > _lib.MAX_CUTOFF_V:
+            raise ValueError("cubic_phase supports cutoff_dim <= %d" % _lib.MAX_CUTOFF_V)
+        self._record("V", [self._remap_modes(mode)], [gamma])
+
     # ------------------------------------------------------------ execution
     def _plan(self, ops, mixed_in):
         key = (tuple((n, m) for n, m, _ in ops), self._n, self._D, mixed_in, self._dtype, str(self._device))
@@ -314,5 +339,4 @@ class HIPFockBackend(BaseFock):
     def _not_here(self, *args, **kwargs):
         raise NotImplementedError("not implemented by the HIP Fock backend")
 
-    mzgate = two_mode_squeeze = cross_kerr_interaction = cubic_phase = _not_here
     thermal_loss = measure_homodyne = measure_fock = measure_threshold = prepare_gkp = _not_here

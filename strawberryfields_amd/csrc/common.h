@@ -156,6 +156,12 @@ int pair_diag(const void* in, void* out, int64_t N, int D, int64_t Bbra,
               int64_t Bket, const void* d, int dtype, hipStream_t s);
 int pair_bs(const void* in, void* out, int64_t N, int D, int64_t sp, int64_t sq,
             const void* blocks, int conj, int dtype, hipStream_t s);
+int pair_s2(const void* in, void* out, int64_t N, int D, int64_t sp, int64_t sq,
+            const void* blocks, int conj, int dtype, hipStream_t s);
+int pair_s2_adj(const void* psi, const void* lam_out, void* lam_in, int64_t N,
+                int D, int64_t sp, int64_t sq, const void* blocks,
+                const void* dblocks, int conj, double* partial, int nwg, int zp1,
+                int dtype, hipStream_t s);
 int loss_fwd(const void* in, void* out, int64_t N, int D, int64_t sp,
              int64_t sq, const void* coef, int dtype, hipStream_t s);
 int mix_fwd(const void* ket, void* dm, int n, int D, int dtype, hipStream_t s);
@@ -175,10 +181,12 @@ int axis_diag_adj(const void* psi, const void* lam_out, void* lam_in, int64_t N,
 int pair_diag_adj(const void* psi, const void* lam_out, void* lam_in, int64_t N,
                   int D, int64_t Bbra, int64_t Bket, const void* d, int kerr,
                   double* partial, int nwg, int zp1, int dtype, hipStream_t s);
+// beamsplitter-family adjoint: the second parameter's derivative is
+// i sigma (wa n + wb N) G[N][n] (BS: wa = 1, wb = -1; MZ: 1, 0; CK: 0, 0)
 int pair_bs_adj(const void* psi, const void* lam_out, void* lam_in, int64_t N,
                 int D, int64_t sp, int64_t sq, const void* blocks,
-                const void* dblocks, int conj, double* partial, int nwg, int zp1,
-                int dtype, hipStream_t s);
+                const void* dblocks, int conj, int wa, int wb, double* partial,
+                int nwg, int zp1, int dtype, hipStream_t s);
 int loss_adj(const void* psi, const void* lam_out, void* lam_in, int64_t N,
              int D, int64_t sp, int64_t sq, const void* coef, const void* dcoef,
              double* partial, int nwg, int zp1, int dtype, hipStream_t s);

@@ -21,8 +21,8 @@ int axis_dense_adj_fib(const void* psi, const void* lam_out, void* lam_in, int64
 int pair_bs_fib(const void* in, void* out, int64_t N, int D, int64_t sp, int64_t sq, const void* blocks, int conj,
                 int dtype, hipStream_t s);
 int pair_bs_adj_fib(const void* psi, const void* lam_out, void* lam_in, int64_t N, int D, int64_t sp, int64_t sq,
-                    const void* blocks, const void* dblocks, int conj, double* partial, int nwg, int zp1, int dtype,
-                    hipStream_t s);
+                    const void* blocks, const void* dblocks, int conj, int wa, int wb, double* partial, int nwg,
+                    int zp1, int dtype, hipStream_t s);
 int loss_fib(const void* in, void* out, int64_t N, int D, int64_t B, const void* coef, int dtype, hipStream_t s);
 int loss_adj_fib(const void* psi, const void* lam_out, void* lam_in, int64_t N, int D, int64_t B, const void* coef,
                  const void* dcoef, double* partial, int nwg, int zp1, int dtype, hipStream_t s);

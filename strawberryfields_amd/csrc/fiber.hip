@@ -177,7 +177,7 @@ template <typename T, int D>
 __global__ void __launch_bounds__(256)
 k_pair_bs_adj_fib(const C<T>* __restrict__ psi, const C<T>* __restrict__ lo_, C<T>* __restrict__ li, int64_t NF,
                   Geo2 g, int64_t sp, int64_t sq, const C<T>* __restrict__ blk, const C<T>* __restrict__ dblk,
-                  int conj, double* partial, int nwg, int zp1) {
+                  int conj, int wa, int wb, double* partial, int nwg, int zp1) {
   constexpr int NSB = 2 * D - 1;
   const T sg = conj ? (T)-1 : (T)1;
   const int64_t ntask = (NF + kBlock - 1) / kBlock * NSB;
@@ -204,7 +204,7 @@ k_pair_bs_adj_fib(const C<T>* __restrict__ psi, const C<T>* __restrict__ lo_, C<
     for (int n = 0; n < D; ++n) {
       if (n >= c) continue;
       // lam_in[n] = sum_N conj(B'[N][n]) lam[N];  w0 = sum_N conj(lam[N]) dB'[N][n];
-      // w1 = sum_N conj(lam[N]) i sg (n - N) B'[N][n]   (indices relative to lo)
+      // w1 = sum_N conj(lam[N]) i sg (wa n + wb N) B'[N][n]   (n, N relative to lo)
       C<T> acc = mk<T>(0, 0), w0 = mk<T>(0, 0), w1 = mk<T>(0, 0);
 #pragma unroll
       for (int N = 0; N < D; ++N) {
@@ -218,7 +218,7 @@ k_pair_bs_adj_fib(const C<T>* __restrict__ psi, const C<T>* __restrict__ lo_, C<
         cmac_conj(acc, b, l[N]);
         cmac_conj(w0, l[N], d);
         const C<T> lb = mk<T>(l[N].re * b.re + l[N].im * b.im, l[N].re * b.im - l[N].im * b.re);
-        const T wt = sg * (T)(n - N);
+        const T wt = sg * (T)(wa * (lo + n) + wb * (lo + N));
         w1.re += -wt * lb.im;
         w1.im += wt * lb.re;
       }
@@ -426,8 +426,8 @@ int pair_bs_fib(const void* in, void* out, int64_t N, int D, int64_t sp, int64_t
 }
 
 int pair_bs_adj_fib(const void* psi, const void* lam_out, void* lam_in, int64_t N, int D, int64_t sp, int64_t sq,
-                    const void* blocks, const void* dblocks, int conj, double* partial, int nwg, int zp1, int dtype,
-                    hipStream_t s) {
+                    const void* blocks, const void* dblocks, int conj, int wa, int wb, double* partial, int nwg,
+                    int zp1, int dtype, hipStream_t s) {
   const int64_t NF = N / ((int64_t)D * D);
   const Geo2 g = geo2(sp, sq, N, D);
   switch (D) {
@@ -436,11 +436,11 @@ int pair_bs_adj_fib(const void* psi, const void* lam_out, void* lam_in, int64_t 
     if (dtype == SFAMD_C64)                                                                                          \
       hipLaunchKernelGGL((k_pair_bs_adj_fib<float, d>), dim3(nwg), dim3(kBlock), 0, s, CP(float, psi),             \
                          CP(float, lam_out), MP(float, lam_in), NF, g, sp, sq, CP(float, blocks), CP(float, dblocks), \
-                         conj, partial, nwg, zp1);                                                                   \
+                         conj, wa, wb, partial, nwg, zp1);                                                           \
     else                                                                                                             \
       hipLaunchKernelGGL((k_pair_bs_adj_fib<double, d>), dim3(nwg), dim3(kBlock), 0, s, CP(double, psi),           \
                          CP(double, lam_out), MP(double, lam_in), NF, g, sp, sq, CP(double, blocks),               \
-                         CP(double, dblocks), conj, partial, nwg, zp1);                                              \
+                         CP(double, dblocks), conj, wa, wb, partial, nwg, zp1);                                      \
     break;
     SFAMD_FIB_D(CASE)
 #undef CASE

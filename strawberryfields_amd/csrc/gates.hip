@@ -201,14 +201,22 @@ __device__ void build_sq(double r, double phi, int D, cpx<T>* out, cd (*col3)[3]
 
 constexpr int kBSMax = SFAMD_MAX_CUTOFF_BS;
 constexpr int kBSPerThread = (kBSMax * kBSMax + 255) / 256;
+constexpr int kMax3 = (2 * kBSMax * kBSMax * 2 > 7 * SFAMD_MAX_CUTOFF_V * SFAMD_MAX_CUTOFF_V)
+                          ? 2 * kBSMax * kBSMax * 2
+                          : 7 * SFAMD_MAX_CUTOFF_V * SFAMD_MAX_CUTOFF_V;
+constexpr int kBuildLdsDoubles = kMax3 > 3 * 3 * SFAMD_MAX_CUTOFF * 2 ? kMax3 : 3 * 3 * SFAMD_MAX_CUTOFF * 2;
 
-// beamsplitter: photon-number blocks of B then of dB/dtheta
+// Passive two-mode gates (BS, MZ): the gate maps a_j^dag -> sum_i W[i][j] a_i^dag
+// for a 2x2 unitary W (BS: W = [[cos, -e^{-i phi} sin], [e^{i phi} sin, cos]]),
+// so G a2^dag = (W[0][1] a1^dag + W[1][1] a2^dag) G and
+//    G a1^dag = (W[0][0] a1^dag + W[1][0] a2^dag) G,
+// which links photon-number block s to block s - 1.  dW carries the derivative
+// with respect to the first parameter through the same recurrence.
+// Output: blocks of G, then blocks of dG/dp0.
 template <typename T>
-__device__ void build_bs(double theta, double phi, int D, cpx<T>* out, cd (*prev)[kBSMax * kBSMax]) {
+__device__ void build_passive(const cd (&W)[2][2], const cd (&dW)[2][2], int D, cpx<T>* out,
+                              cd (*prev)[kBSMax * kBSMax]) {
   const int64_t nb = bs_nb(D);
-  const double ct = cos(theta), st = sin(theta);
-  const cd e = mk<double>(cos(phi), sin(phi));
-  const cd ec = cconj(e);
   if (threadIdx.x == 0) {
     prev[0][0] = mk<double>(1.0, 0.0);
     prev[1][0] = mk<double>(0.0, 0.0);
@@ -226,47 +234,25 @@ __device__ void build_bs(double theta, double phi, int D, cpx<T>* out, cd (*prev
       const int N = lo + idx / c, n = lo + idx % c;
       const int M = s - N, m = s - n;
       cd a = mk<double>(0.0, 0.0), da = mk<double>(0.0, 0.0);
-      if (m > 0) {
-        // (ct sqrt(M) P[N][n] - ec st sqrt(N) P[N-1][n]) / sqrt(m)
-        if (M >= 1) {
-          const int pi = (N - plo) * pc + (n - plo);
-          const double sM = sqrt((double)M);
-          a = cadd(a, cscale(prev[0][pi], ct * sM));
-          da = cadd(da, cscale(prev[0][pi], -st * sM));
-          da = cadd(da, cscale(prev[1][pi], ct * sM));
-        }
-        if (N >= 1) {
-          const int pi = (N - 1 - plo) * pc + (n - plo);
-          const double sN = sqrt((double)N);
-          a = csub(a, cscale(cmul(ec, prev[0][pi]), st * sN));
-          da = csub(da, cscale(cmul(ec, prev[0][pi]), ct * sN));
-          da = csub(da, cscale(cmul(ec, prev[1][pi]), st * sN));
-        }
-        const double is = 1.0 / sqrt((double)m);
-        a = cscale(a, is);
-        da = cscale(da, is);
-      } else {
-        // n == s: (ct sqrt(N) P[N-1][s-1] + e st sqrt(M) P[N][s-1]) / sqrt(s)
-        if (N >= 1) {
-          const int pi = (N - 1 - plo) * pc + (s - 1 - plo);
-          const double sN = sqrt((double)N);
-          a = cadd(a, cscale(prev[0][pi], ct * sN));
-          da = cadd(da, cscale(prev[0][pi], -st * sN));
-          da = cadd(da, cscale(prev[1][pi], ct * sN));
-        }
-        if (M >= 1) {
-          const int pi = (N - plo) * pc + (s - 1 - plo);
-          const double sM = sqrt((double)M);
-          a = cadd(a, cscale(cmul(e, prev[0][pi]), st * sM));
-          da = cadd(da, cscale(cmul(e, prev[0][pi]), ct * sM));
-          da = cadd(da, cscale(cmul(e, prev[1][pi]), st * sM));
-        }
-        const double is = 1.0 / sqrt((double)s);
-        a = cscale(a, is);
-        da = cscale(da, is);
+      // input (n, m): raise mode 2 from (n, m-1) if m > 0, else mode 1 from (s-1, 0)
+      const int pn = m > 0 ? n : s - 1;
+      const cd w1 = m > 0 ? W[0][1] : W[0][0], w2 = m > 0 ? W[1][1] : W[1][0];
+      const cd dw1 = m > 0 ? dW[0][1] : dW[0][0], dw2 = m > 0 ? dW[1][1] : dW[1][0];
+      if (N >= 1) {  // w1 sqrt(N) P[N-1][pn]
+        const int pi = (N - 1 - plo) * pc + (pn - plo);
+        const double sN = sqrt((double)N);
+        a = cadd(a, cscale(cmul(w1, prev[0][pi]), sN));
+        da = cadd(da, cscale(cadd(cmul(dw1, prev[0][pi]), cmul(w1, prev[1][pi])), sN));
       }
-      v[cnt] = a;
-      dv[cnt] = da;
+      if (M >= 1) {  // w2 sqrt(M) P[N][pn]
+        const int pi = (N - plo) * pc + (pn - plo);
+        const double sM = sqrt((double)M);
+        a = cadd(a, cscale(cmul(w2, prev[0][pi]), sM));
+        da = cadd(da, cscale(cadd(cmul(dw2, prev[0][pi]), cmul(w2, prev[1][pi])), sM));
+      }
+      const double is = 1.0 / sqrt((double)(m > 0 ? m : s));
+      v[cnt] = cscale(a, is);
+      dv[cnt] = cscale(da, is);
     }
     __syncthreads();
     cnt = 0;
@@ -277,6 +263,231 @@ __device__ void build_bs(double theta, double phi, int D, cpx<T>* out, cd (*prev
       put(out, nb + off + idx, dv[cnt]);
     }
     __syncthreads();
+  }
+}
+
+// beamsplitter BS(theta, phi): blocks of B then of dB/dtheta
+template <typename T>
+__device__ void build_bs(double theta, double phi, int D, cpx<T>* out, cd (*prev)[kBSMax * kBSMax]) {
+  const double ct = cos(theta), st = sin(theta);
+  const cd e = cexpi(phi), ec = cconj(e);
+  const cd W[2][2] = {{mk<double>(ct, 0.0), cscale(ec, -st)}, {cscale(e, st), mk<double>(ct, 0.0)}};
+  const cd dW[2][2] = {{mk<double>(-st, 0.0), cscale(ec, -ct)}, {cscale(e, ct), mk<double>(-st, 0.0)}};
+  build_passive<T>(W, dW, D, out, prev);
+}
+
+// Mach-Zehnder MZ(phi_in, phi_ex) = BS(pi/4, pi/2) (R(phi_in) x 1) BS(pi/4, pi/2)
+// (R(phi_ex) x 1) (strawberryfields/ops.py:1951-1991): W = B Rin B Rex with
+// B = [[c, i c], [i c, c]], c = 1/sqrt 2.  Blocks of MZ, then dMZ/dphi_in.
+template <typename T>
+__device__ void build_mz(double phi_in, double phi_ex, int D, cpx<T>* out, cd (*prev)[kBSMax * kBSMax]) {
+  const cd ei = cexpi(phi_in), ee = cexpi(phi_ex);
+  const cd I = mk<double>(0.0, 1.0);
+  // B diag(x, 1) B diag(ee, 1) = 1/2 [[(x - 1) ee, i (x + 1)], [i (x + 1) ee, 1 - x]]
+  const cd one = mk<double>(1.0, 0.0);
+  cd W[2][2], dW[2][2];
+  W[0][0] = cscale(cmul(csub(ei, one), ee), 0.5);
+  W[0][1] = cscale(cmul(I, cadd(ei, one)), 0.5);
+  W[1][0] = cscale(cmul(cmul(I, cadd(ei, one)), ee), 0.5);
+  W[1][1] = cscale(csub(one, ei), 0.5);
+  const cd dx = cmul(I, ei);  // d(e^{i phi_in})/dphi_in
+  dW[0][0] = cscale(cmul(dx, ee), 0.5);
+  dW[0][1] = cscale(cmul(I, dx), 0.5);
+  dW[1][0] = cscale(cmul(cmul(I, dx), ee), 0.5);
+  dW[1][1] = cscale(dx, -0.5);
+  build_passive<T>(W, dW, D, out, prev);
+}
+
+// cross-Kerr CK(kappa) = exp(i kappa n1 n2): diagonal blocks, then dCK/dkappa
+template <typename T>
+__device__ void build_ck(double kappa, int D, cpx<T>* out) {
+  const int64_t nb = bs_nb(D);
+  for (int s = 0; s <= 2 * D - 2; ++s) {
+    const int lo = bs_lo(s, D), c = bs_hi(s, D) - lo + 1;
+    const int64_t off = bs_block_off(s, D);
+    for (int idx = threadIdx.x; idx < c * c; idx += blockDim.x) {
+      const int N = lo + idx / c, n = lo + idx % c;
+      cd v = mk<double>(0.0, 0.0), dv = v;
+      if (N == n) {
+        const double w = (double)N * (double)(s - N);
+        v = cexpi(kappa * w);
+        dv = mk<double>(-w * v.im, w * v.re);
+      }
+      put(out, off + idx, v);
+      put(out, nb + off + idx, dv);
+    }
+  }
+}
+
+// two-mode squeezing S2(r, phi) = exp(r (e^{i phi} a1^dag a2^dag - e^{-i phi} a1 a2))
+// (strawberryfields/ops.py:2036-2054).  From S2^dag a1 S2 = cosh r a1 + e^{i phi}
+// sinh r a2^dag (and 1 <-> 2) the entries G[N,M,n,m] = <N,M|S2|n,m> obey
+//   sqrt(N) G[N,M,n,m] = sech sqrt(n) G[N-1,M,n-1,m] + e^{i phi} tanh sqrt(M) G[N-1,M-1,n,m]
+//   sqrt(M) G[0,M,n,m] = sech sqrt(m) G[0,M-1,n,m-1]
+//   sqrt(n) G[0,0,n,n] = -e^{-i phi} tanh sqrt(n) G[0,0,n-1,n-1]
+// with G[0,0,0,0] = sech r.  Every coefficient is at most 1 in magnitude (no
+// cancelling alternating sums), and each right-hand side lies one level lower
+// in lambda = N + m = M + n, so a level is computed from the previous one
+// alone: level lambda is the grid of (N, M) with n = lambda - M, m = lambda - N
+// in range, kept in LDS (value and d/dr) between levels.  Output: blocks
+// s = D - 1 + d (d = N - M = n - m) in the BS layout (first index N, second n),
+// then the blocks of dS2/dr.
+template <typename T>
+__device__ void build_s2(double r, double phi, int D, cpx<T>* out, cd (*prev)[kBSMax * kBSMax]) {
+  const int64_t nb = bs_nb(D);
+  const double sech = 1.0 / cosh(r), th = tanh(r);
+  const cd e = cexpi(phi), ec = cconj(e);
+  const int DD = D * D;
+  if (threadIdx.x == 0) {
+    prev[0][0] = mk<double>(sech, 0.0);
+    prev[1][0] = mk<double>(-sech * th, 0.0);
+    const int64_t off = bs_block_off(D - 1, D);  // d = 0, entry (0, 0)
+    put(out, off, prev[0][0]);
+    put(out, nb + off, prev[1][0]);
+  }
+  __syncthreads();
+  for (int lam = 1; lam <= 2 * D - 2; ++lam) {
+    cd v[kBSPerThread], dv[kBSPerThread];
+    int cnt = 0;
+    for (int idx = threadIdx.x; idx < DD; idx += blockDim.x, ++cnt) {
+      const int N = idx / D, M = idx % D, n = lam - M, m = lam - N;
+      v[cnt] = dv[cnt] = mk<double>(0.0, 0.0);
+      if (n < 0 || n >= D || m < 0 || m >= D) continue;
+      cd a = mk<double>(0.0, 0.0), da = a;
+      if (N > 0) {
+        if (n > 0) {  // sech sqrt(n) G[N-1, M, n-1, m]
+          const cd g = prev[0][(N - 1) * D + M], dg = prev[1][(N - 1) * D + M];
+          const double c = sech * sqrt((double)n);
+          a = cadd(a, cscale(g, c));
+          da = cadd(da, cscale(csub(dg, cscale(g, th)), c));
+        }
+        if (M > 0) {  // e tanh sqrt(M) G[N-1, M-1, n, m]
+          const cd g = prev[0][(N - 1) * D + M - 1], dg = prev[1][(N - 1) * D + M - 1];
+          const double sM = sqrt((double)M);
+          a = cadd(a, cscale(cmul(e, g), th * sM));
+          da = cadd(da, cscale(cmul(e, cadd(cscale(dg, th), cscale(g, sech * sech))), sM));
+        }
+        const double is = 1.0 / sqrt((double)N);
+        a = cscale(a, is);
+        da = cscale(da, is);
+      } else if (M > 0) {  // N = 0, so m = lam >= M >= 1
+        const cd g = prev[0][M - 1], dg = prev[1][M - 1];
+        const double c = sech * sqrt((double)m) / sqrt((double)M);
+        a = cscale(g, c);
+        da = cscale(csub(dg, cscale(g, th)), c);
+      } else {  // N = M = 0, n = m = lam
+        const cd g = prev[0][0], dg = prev[1][0];
+        a = cscale(cmul(ec, g), -th);
+        da = cscale(cmul(ec, cadd(cscale(dg, th), cscale(g, sech * sech))), -1.0);
+      }
+      v[cnt] = a;
+      dv[cnt] = da;
+    }
+    __syncthreads();
+    cnt = 0;
+    for (int idx = threadIdx.x; idx < DD; idx += blockDim.x, ++cnt) {
+      const int N = idx / D, M = idx % D, n = lam - M, m = lam - N;
+      if (n < 0 || n >= D || m < 0 || m >= D) continue;
+      prev[0][idx] = v[cnt];
+      prev[1][idx] = dv[cnt];
+      const int s = D - 1 + N - M, lo = bs_lo(s, D), c = bs_hi(s, D) - lo + 1;
+      const int64_t at = bs_block_off(s, D) + (int64_t)(N - lo) * c + (n - lo);
+      put(out, at, v[cnt]);
+      put(out, nb + at, dv[cnt]);
+    }
+    __syncthreads();
+  }
+}
+
+// cubic phase V(gamma) = expm(i gamma/(3 hbar) x^3), hbar = 2, x = a + a^dag on
+// the truncated basis (fockbackend/ops.py:297-306 builds x^3 from the truncated
+// x and exponentiates it; the gate is therefore that matrix exponential, not a
+// truncation of the exact operator).  H = (gamma / 6) x^3 is real symmetric;
+// exp(iH) by scaling and squaring of a degree-20 Taylor polynomial in FP64, and
+// dV/dgamma = (i x^3 / 6) V.  Output: V, dV/dgamma, zeros (row = out index).
+template <typename T>
+__device__ void build_cubic(double gamma, int D, cpx<T>* out, double* sm) {
+  // LDS: X3 (real D*D), E, P, Q (complex D*D each)
+  double* X3 = sm;
+  cd* E = reinterpret_cast<cd*>(sm + D * D);
+  cd* P = E + D * D;
+  cd* Q = P + D * D;
+  const int DD = D * D;
+  // x^3: x[i][j] = sqrt(j) [i == j-1] + sqrt(i) [i == j+1];  x^3 = x (x x)
+  for (int e = threadIdx.x; e < DD; e += blockDim.x) {
+    const int i = e / D, j = e % D;
+    double x2 = 0.0;  // (x x)[i][j]
+    for (int k = 0; k < D; ++k) {
+      const double xik = (k == i + 1 ? sqrt((double)k) : 0.0) + (k + 1 == i ? sqrt((double)i) : 0.0);
+      const double xkj = (j == k + 1 ? sqrt((double)j) : 0.0) + (j + 1 == k ? sqrt((double)k) : 0.0);
+      x2 += xik * xkj;
+    }
+    P[e] = mk<double>(x2, 0.0);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < DD; e += blockDim.x) {
+    const int i = e / D, j = e % D;
+    double v = 0.0;
+    if (i > 0) v += sqrt((double)i) * P[(i - 1) * D + j].re;
+    if (i + 1 < D) v += sqrt((double)(i + 1)) * P[(i + 1) * D + j].re;
+    X3[e] = v;
+  }
+  __syncthreads();
+  // scale: ||H||_1 / 2^k <= 1/2
+  __shared__ double norm1;
+  if (threadIdx.x == 0) {
+    double mx = 0.0;
+    for (int j = 0; j < D; ++j) {
+      double cs = 0.0;
+      for (int i = 0; i < D; ++i) cs += fabs(X3[i * D + j]);
+      mx = cs > mx ? cs : mx;
+    }
+    norm1 = mx * fabs(gamma) / 6.0;
+  }
+  __syncthreads();
+  int k = 0;  // bounded: a non-finite gamma stops at once and gives NaN entries
+  while (k < 64 && norm1 / ldexp(1.0, k) > 0.5) ++k;
+  const double h = gamma / 6.0 / ldexp(1.0, k);  // A = i h X3
+  // E = sum_{m <= 20} A^m / m!  (P holds A^m / m!)
+  for (int e = threadIdx.x; e < DD; e += blockDim.x) {
+    const int i = e / D, j = e % D;
+    E[e] = mk<double>(i == j ? 1.0 : 0.0, 0.0);
+    P[e] = E[e];
+  }
+  __syncthreads();
+  for (int m = 1; m <= 20; ++m) {
+    for (int e = threadIdx.x; e < DD; e += blockDim.x) {
+      const int i = e / D, j = e % D;
+      cd acc = mk<double>(0.0, 0.0);
+      for (int q = 0; q < D; ++q) acc = cadd(acc, cscale(P[i * D + q], X3[q * D + j]));
+      // times i h / m
+      Q[e] = mk<double>(-acc.im * h / m, acc.re * h / m);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < DD; e += blockDim.x) {
+      P[e] = Q[e];
+      E[e] = cadd(E[e], Q[e]);
+    }
+    __syncthreads();
+  }
+  for (int it = 0; it < k; ++it) {  // E <- E E
+    for (int e = threadIdx.x; e < DD; e += blockDim.x) {
+      const int i = e / D, j = e % D;
+      cd acc = mk<double>(0.0, 0.0);
+      for (int q = 0; q < D; ++q) acc = cadd(acc, cmul(E[i * D + q], E[q * D + j]));
+      Q[e] = acc;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < DD; e += blockDim.x) E[e] = Q[e];
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < DD; e += blockDim.x) {
+    const int i = e / D, j = e % D;
+    cd acc = mk<double>(0.0, 0.0);  // (x^3 / 6) V
+    for (int q = 0; q < D; ++q) acc = cadd(acc, cscale(E[q * D + j], X3[i * D + q] / 6.0));
+    put(out, e, E[e]);
+    put(out, DD + e, mk<double>(-acc.im, acc.re));
+    put(out, 2 * DD + e, mk<double>(0.0, 0.0));
   }
 }
 
@@ -314,8 +525,11 @@ __device__ void build_loss(double Tt, int D, cpx<T>* out) {
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_build(sfamd::BuildBatch b, int D, const double* params, cpx<T>* table) {
-  __shared__ cd sm_bs[2][kBSMax * kBSMax];
-  __shared__ cd sm_col[3][3][SFAMD_MAX_CUTOFF];
+  // one LDS buffer serves every builder: passive-gate blocks, D/S columns, or
+  // the cubic-phase matrices (x^3 real, three complex D x D)
+  __shared__ __align__(16) double sm_raw[kBuildLdsDoubles];
+  cd(*sm_bs)[kBSMax * kBSMax] = reinterpret_cast<cd(*)[kBSMax * kBSMax]>(sm_raw);
+  cd(*sm_col)[3][SFAMD_MAX_CUTOFF] = reinterpret_cast<cd(*)[3][SFAMD_MAX_CUTOFF]>(sm_raw);
   const sfamd::BuildDesc d = b.d[blockIdx.x];
   double p0 = b.p0, p1 = b.p1;
   if (b.first >= 0) {
@@ -338,6 +552,18 @@ __global__ void __launch_bounds__(256) k_build(sfamd::BuildBatch b, int D, const
     case SFAMD_OP_BEAMSPLITTER:
       build_bs<T>(p0, p1, D, out, sm_bs);
       break;
+    case SFAMD_OP_MZGATE:
+      build_mz<T>(p0, p1, D, out, sm_bs);
+      break;
+    case SFAMD_OP_CROSS_KERR:
+      build_ck<T>(p0, D, out);
+      break;
+    case SFAMD_OP_TWO_MODE_SQUEEZE:
+      build_s2<T>(p0, p1, D, out, sm_bs);
+      break;
+    case SFAMD_OP_CUBIC_PHASE:
+      build_cubic<T>(p0, D, out, sm_raw);
+      break;
     case SFAMD_OP_LOSS:
       build_loss<T>(p0, D, out);
       break;
@@ -359,7 +585,12 @@ int64_t table_size(int kind, int D) {
     case SFAMD_OP_SQUEEZE:
       return 3 * (int64_t)D * D;
     case SFAMD_OP_BEAMSPLITTER:
+    case SFAMD_OP_TWO_MODE_SQUEEZE:
+    case SFAMD_OP_MZGATE:
+    case SFAMD_OP_CROSS_KERR:
       return 2 * bs_nb(D);
+    case SFAMD_OP_CUBIC_PHASE:
+      return 3 * (int64_t)D * D;
     case SFAMD_OP_LOSS:
       return (int64_t)D * D * D;
     case SFAMD_OP_MIX:

@@ -142,6 +142,33 @@ k_pair_bs(const C<T>* __restrict__ in, C<T>* __restrict__ out, int64_t N, int D,
   }
 }
 
+// two-mode squeezing on axes p (stride sp, first pair of the matrix) and q:
+// out1 - out2 = in1 - in2 = d, so out[i, j] = sum_k G_d[i][k] in[k, k-d] with
+// d = i - j; block d is stored as block s = D - 1 + d of the compact layout
+// (first index i = out1, second k = in1, both from bs_lo(s) to bs_hi(s)).
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_pair_s2(const C<T>* __restrict__ in, C<T>* __restrict__ out, int64_t N, int D, int64_t sp, int64_t sq,
+          const C<T>* __restrict__ blk, int conj) {
+  GRID_STRIDE(e, N) {
+    const int i = (int)((e / sp) % D);
+    const int j = (int)((e / sq) % D);
+    const int d = i - j, s = D - 1 + d;
+    const int lo = bs_lo(s, D), hi = bs_hi(s, D), c = hi - lo + 1;
+    const int64_t base = e - (int64_t)i * sp - (int64_t)j * sq;
+    const C<T>* row = blk + bs_block_off(s, D) + (int64_t)(i - lo) * c;
+    C<T> acc = mk<T>(0, 0);
+    for (int k = lo; k <= hi; ++k) {
+      const C<T> x = in[base + (int64_t)k * sp + (int64_t)(k - d) * sq];
+      if (conj)
+        cmac_conj(acc, row[k - lo], x);
+      else
+        cmac(acc, row[k - lo], x);
+    }
+    out[e] = acc;
+  }
+}
+
 // loss on the (bra, ket) axes of one mode: out[p, q] = sum_l c[p][q][l] in[p+l, q+l]
 template <typename T>
 __global__ void __launch_bounds__(256)
@@ -338,7 +365,7 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 k_pair_bs_adj(const C<T>* __restrict__ psi, const C<T>* __restrict__ lo_, C<T>* __restrict__ li, int64_t N,
               int D, int64_t sp, int64_t sq, const C<T>* __restrict__ blk, const C<T>* __restrict__ dblk,
-              int conj, double* partial, int nwg, int zp1) {
+              int conj, int wa, int wb, double* partial, int nwg, int zp1) {
   double g0 = 0.0, g1 = 0.0;
   GRID_STRIDE(e, N) {
     const int i = (int)((e / sp) % D);
@@ -358,7 +385,7 @@ k_pair_bs_adj(const C<T>* __restrict__ psi, const C<T>* __restrict__ lo_, C<T>* 
         cmac_conj(acc, b, l);
     }
     li[e] = acc;
-    // grads: d/dtheta from the derivative blocks, d/dphi B'[i][k] = i sigma (k - i) B'[i][k]
+    // grads: d/dp0 from the derivative blocks, d/dp1 B'[i][k] = i sigma (wa k + wb i) B'[i][k]
     C<T> ut = mk<T>(0, 0), up = mk<T>(0, 0);
     const C<T>* row = blk + off + (int64_t)(i - lo) * c;
     const C<T>* drow = dblk + off + (int64_t)(i - lo) * c;
@@ -370,7 +397,55 @@ k_pair_bs_adj(const C<T>* __restrict__ psi, const C<T>* __restrict__ lo_, C<T>* 
         db.im = -db.im;
       }
       cmac(ut, db, x);
-      const T w = (T)(k - i) * (conj ? (T)-1 : (T)1);
+      const T w = (T)(wa * k + wb * i) * (conj ? (T)-1 : (T)1);
+      const C<T> bx = cmul(b, x);
+      up.re += -w * bx.im;
+      up.im += w * bx.re;
+    }
+    const C<T> l = lo_[e];
+    g0 += (double)re_conj_mul(l, ut);
+    g1 += (double)re_conj_mul(l, up);
+  }
+  write_partials(g0, g1, partial, nwg, zp1);
+}
+
+// adjoint of k_pair_s2: lam_in[(i, i-d)] = sum_N conj(G'[N][i]) lam_out[(N, N-d)];
+// gradients: d/dr from the derivative blocks, d/dphi G'[N][k] = i sigma (N - k) G'[N][k]
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_pair_s2_adj(const C<T>* __restrict__ psi, const C<T>* __restrict__ lo_, C<T>* __restrict__ li, int64_t N, int D,
+              int64_t sp, int64_t sq, const C<T>* __restrict__ blk, const C<T>* __restrict__ dblk, int conj,
+              double* partial, int nwg, int zp1) {
+  double g0 = 0.0, g1 = 0.0;
+  GRID_STRIDE(e, N) {
+    const int i = (int)((e / sp) % D);
+    const int j = (int)((e / sq) % D);
+    const int d = i - j, s = D - 1 + d;
+    const int lo = bs_lo(s, D), hi = bs_hi(s, D), c = hi - lo + 1;
+    const int64_t base = e - (int64_t)i * sp - (int64_t)j * sq;
+    const int64_t off = bs_block_off(s, D);
+    C<T> acc = mk<T>(0, 0);
+    for (int Nn = lo; Nn <= hi; ++Nn) {
+      const C<T> b = blk[off + (int64_t)(Nn - lo) * c + (i - lo)];
+      const C<T> l = lo_[base + (int64_t)Nn * sp + (int64_t)(Nn - d) * sq];
+      if (conj)
+        cmac(acc, b, l);
+      else
+        cmac_conj(acc, b, l);
+    }
+    li[e] = acc;
+    C<T> ut = mk<T>(0, 0), up = mk<T>(0, 0);
+    const C<T>* row = blk + off + (int64_t)(i - lo) * c;
+    const C<T>* drow = dblk + off + (int64_t)(i - lo) * c;
+    for (int k = lo; k <= hi; ++k) {
+      const C<T> x = psi[base + (int64_t)k * sp + (int64_t)(k - d) * sq];
+      C<T> b = row[k - lo], db = drow[k - lo];
+      if (conj) {
+        b.im = -b.im;
+        db.im = -db.im;
+      }
+      cmac(ut, db, x);
+      const T w = (T)(i - k) * (conj ? (T)-1 : (T)1);
       const C<T> bx = cmul(b, x);
       up.re += -w * bx.im;
       up.im += w * bx.re;
@@ -817,6 +892,30 @@ int pair_bs(const void* in, void* out, int64_t N, int D, int64_t sp, int64_t sq,
   return (int)hipGetLastError();
 }
 
+int pair_s2(const void* in, void* out, int64_t N, int D, int64_t sp, int64_t sq, const void* blocks, int conj,
+            int dtype, hipStream_t s) {
+  if (dtype == SFAMD_C64)
+    hipLaunchKernelGGL(k_pair_s2<float>, dim3(grid_for(N)), dim3(kBlock), 0, s, CP(float, in), MP(float, out), N, D,
+                       sp, sq, CP(float, blocks), conj);
+  else
+    hipLaunchKernelGGL(k_pair_s2<double>, dim3(grid_for(N)), dim3(kBlock), 0, s, CP(double, in), MP(double, out), N,
+                       D, sp, sq, CP(double, blocks), conj);
+  return (int)hipGetLastError();
+}
+
+int pair_s2_adj(const void* psi, const void* lam_out, void* lam_in, int64_t N, int D, int64_t sp, int64_t sq,
+                const void* blocks, const void* dblocks, int conj, double* partial, int nwg, int zp1, int dtype,
+                hipStream_t s) {
+  if (dtype == SFAMD_C64)
+    hipLaunchKernelGGL(k_pair_s2_adj<float>, dim3(nwg), dim3(kBlock), 0, s, CP(float, psi), CP(float, lam_out),
+                       MP(float, lam_in), N, D, sp, sq, CP(float, blocks), CP(float, dblocks), conj, partial, nwg, zp1);
+  else
+    hipLaunchKernelGGL(k_pair_s2_adj<double>, dim3(nwg), dim3(kBlock), 0, s, CP(double, psi), CP(double, lam_out),
+                       MP(double, lam_in), N, D, sp, sq, CP(double, blocks), CP(double, dblocks), conj, partial, nwg,
+                       zp1);
+  return (int)hipGetLastError();
+}
+
 int loss_fwd(const void* in, void* out, int64_t N, int D, int64_t sp, int64_t sq, const void* coef, int dtype,
              hipStream_t s) {
   if (dtype == SFAMD_C64)
@@ -901,14 +1000,17 @@ int pair_diag_adj(const void* psi, const void* lam_out, void* lam_in, int64_t N,
 }
 
 int pair_bs_adj(const void* psi, const void* lam_out, void* lam_in, int64_t N, int D, int64_t sp, int64_t sq,
-                const void* blocks, const void* dblocks, int conj, double* partial, int nwg, int zp1, int dtype,
+                const void* blocks, const void* dblocks, int conj, int wa, int wb, double* partial, int nwg, int zp1,
+                int dtype,
                 hipStream_t s) {
   if (dtype == SFAMD_C64)
     hipLaunchKernelGGL(k_pair_bs_adj<float>, dim3(nwg), dim3(kBlock), 0, s, CP(float, psi), CP(float, lam_out),
-                       MP(float, lam_in), N, D, sp, sq, CP(float, blocks), CP(float, dblocks), conj, partial, nwg, zp1);
+                       MP(float, lam_in), N, D, sp, sq, CP(float, blocks), CP(float, dblocks), conj, wa, wb, partial, nwg,
+                       zp1);
   else
     hipLaunchKernelGGL(k_pair_bs_adj<double>, dim3(nwg), dim3(kBlock), 0, s, CP(double, psi), CP(double, lam_out),
-                       MP(double, lam_in), N, D, sp, sq, CP(double, blocks), CP(double, dblocks), conj, partial, nwg, zp1);
+                       MP(double, lam_in), N, D, sp, sq, CP(double, blocks), CP(double, dblocks), conj, wa, wb, partial,
+                       nwg, zp1);
   return (int)hipGetLastError();
 }
 

@@ -20,8 +20,8 @@ int axis_dense_adj_mfma(const void* psi, const void* lam_out, void* lam_in, int6
 int pair_bs_mfma(const void* in, void* out, int64_t N, int D, int64_t sp, int64_t sq, const void* blocks, int conj,
                  hipStream_t s);
 int pair_bs_adj_mfma(const void* psi, const void* lam_out, void* lam_in, int64_t N, int D, int64_t sp, int64_t sq,
-                     const void* blocks, const void* dblocks, int conj, double* partial, int nwg, int zp1,
-                     hipStream_t s);
+                     const void* blocks, const void* dblocks, int conj, int wa, int wb, double* partial, int nwg,
+                     int zp1, hipStream_t s);
 
 // rho' = U rho U^H on one mode of a DM (bra stride D * sk, ket stride sk) in
 // one pass; supported for sk == 1 or sk >= 16

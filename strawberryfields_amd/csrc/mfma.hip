@@ -241,14 +241,15 @@ k_bs_mfma(const C* __restrict__ in, C* __restrict__ out, int64_t NF, PairGeo geo
   }
 }
 
-// adjoint of one task: lam_in = B'^H lam_out; gradients through dB'/dtheta
-// (table) and dB'/dphi = i sg (n - N) B'[N][n] (sg = -1 on the conjugated
+// adjoint of one task: lam_in = B'^H lam_out; gradients through dB'/dp0
+// (table) and dB'/dp1 = i sg (wa n + wb N) B'[N][n] (sg = -1 on the conjugated
 // side).  opl: the workgroup's LDS for three operators of MB blocks.
 template <int MB, int D>
 __device__ __forceinline__ void bs_adj_task(const C* __restrict__ psi, const C* __restrict__ lo_, C* __restrict__ li,
                                             int64_t NF, const PairGeo& geo, int64_t sp, int64_t sq,
                                             const C* __restrict__ bb, const C* __restrict__ db, int c, int lo, int s,
-                                            int conj, int64_t f0, float* opl, double& g0, double& g1) {
+                                            int conj, int wa, int wb, int64_t f0, float* opl, double& g0,
+                                            double& g1) {
   constexpr int Q = Blk<MB>::Q, OPN = MB * Blk<MB>::NS * 64;
   __syncthreads();  // the previous task's operators are no longer read
   build_op_lds<MB>([&](int i, int k) { return op_entry(bb, c, conj ? 3 : 2, i, k); }, opl);
@@ -257,7 +258,7 @@ __device__ __forceinline__ void bs_adj_task(const C* __restrict__ psi, const C* 
   build_op_lds<MB>(
       [&](int i, int k) {
         const C m = op_entry(bb, c, conj ? 1 : 0, i, k);
-        const float w = sg * (float)(k - i);
+        const float w = sg * (float)(wa * (lo + k) + wb * (lo + i));
         return mk<float>(-w * m.im, w * m.re);
       },
       opl + 2 * OPN);
@@ -301,8 +302,8 @@ __device__ __forceinline__ void bs_adj_task(const C* __restrict__ psi, const C* 
 template <int D>
 __global__ void __launch_bounds__(256)
 k_bs_mfma_adj(const C* __restrict__ psi, const C* __restrict__ lo_, C* __restrict__ li, int64_t NF, PairGeo geo,
-              int64_t sp, int64_t sq, const C* __restrict__ blk, const C* __restrict__ dblk, int conj,
-              double* __restrict__ partial, int nwg, int zp1) {
+              int64_t sp, int64_t sq, const C* __restrict__ blk, const C* __restrict__ dblk, int conj, int wa,
+              int wb, double* __restrict__ partial, int nwg, int zp1) {
   constexpr int NSB = 2 * D - 1, MBX = Plan<D>::Mb;
   __shared__ float opl[3 * MBX * Blk<MBX>::NS * 64];
   const int64_t ntask = (NF + kBsChunk - 1) / kBsChunk * NSB;
@@ -315,12 +316,15 @@ k_bs_mfma_adj(const C* __restrict__ psi, const C* __restrict__ lo_, C* __restric
     const int64_t off = bs_block_off(s, D);
     const int mb = (2 * c + 15) / 16;
     if (mb == 1)
-      bs_adj_task<1, D>(psi, lo_, li, NF, geo, sp, sq, blk + off, dblk + off, c, lo, s, conj, f0, opl, g0, g1);
+      bs_adj_task<1, D>(psi, lo_, li, NF, geo, sp, sq, blk + off, dblk + off, c, lo, s, conj, wa, wb, f0, opl, g0,
+                          g1);
     else if constexpr (D > 8) {
       if (mb == 2)
-        bs_adj_task<2, D>(psi, lo_, li, NF, geo, sp, sq, blk + off, dblk + off, c, lo, s, conj, f0, opl, g0, g1);
+        bs_adj_task<2, D>(psi, lo_, li, NF, geo, sp, sq, blk + off, dblk + off, c, lo, s, conj, wa, wb, f0, opl, g0,
+                          g1);
       else if constexpr (D > 16)
-        bs_adj_task<3, D>(psi, lo_, li, NF, geo, sp, sq, blk + off, dblk + off, c, lo, s, conj, f0, opl, g0, g1);
+        bs_adj_task<3, D>(psi, lo_, li, NF, geo, sp, sq, blk + off, dblk + off, c, lo, s, conj, wa, wb, f0, opl, g0,
+                          g1);
     }
   }
   mfma_write_partials(g0, g1, partial, nwg, zp1);
@@ -500,15 +504,16 @@ int pair_bs_mfma(const void* in, void* out, int64_t N, int D, int64_t sp, int64_
 }
 
 int pair_bs_adj_mfma(const void* psi, const void* lam_out, void* lam_in, int64_t N, int D, int64_t sp, int64_t sq,
-                     const void* blocks, const void* dblocks, int conj, double* partial, int nwg, int zp1,
-                     hipStream_t s) {
+                     const void* blocks, const void* dblocks, int conj, int wa, int wb, double* partial, int nwg,
+                     int zp1, hipStream_t s) {
   const int64_t NF = N / ((int64_t)D * D);
   const PairGeo g = pair_geo(sp, sq, D);
   switch (D) {
 #define CASE(d)                                                                                                     \
   case d:                                                                                                           \
     hipLaunchKernelGGL((k_bs_mfma_adj<d>), dim3(nwg), dim3(kBlock), 0, s, (const C*)psi, (const C*)lam_out,        \
-                       (C*)lam_in, NF, g, sp, sq, (const C*)blocks, (const C*)dblocks, conj, partial, nwg, zp1);    \
+                       (C*)lam_in, NF, g, sp, sq, (const C*)blocks, (const C*)dblocks, conj, wa, wb, partial, nwg,  \
+                       zp1);                                                                                       \
     break;
     SFAMD_MFMA_D(CASE)
 #undef CASE

@@ -40,16 +40,29 @@ inline const char* cptr(const void* p, int64_t elems, int dtype) {
   return (const char*)p + elems * elem_size(dtype);
 }
 
+bool two_mode_kind(int kind) {
+  return kind == SFAMD_OP_BEAMSPLITTER || kind == SFAMD_OP_TWO_MODE_SQUEEZE || kind == SFAMD_OP_MZGATE ||
+         kind == SFAMD_OP_CROSS_KERR;
+}
+
 int validate_op(int kind, int m0, int m1, int n, int D) {
-  if (kind < 0 || kind > SFAMD_OP_MIX) return SFAMD_E_KIND;
+  if (kind < 0 || kind > SFAMD_OP_CUBIC_PHASE) return SFAMD_E_KIND;
   if (D < 1 || D > SFAMD_MAX_CUTOFF) return SFAMD_E_CUTOFF;
-  if (kind == SFAMD_OP_BEAMSPLITTER) {
+  if (two_mode_kind(kind)) {
     if (D > SFAMD_MAX_CUTOFF_BS) return SFAMD_E_CUTOFF;
     if (m0 < 0 || m0 >= n || m1 < 0 || m1 >= n || m0 == m1) return SFAMD_E_ARG;
   } else if (kind != SFAMD_OP_MIX) {
+    if (kind == SFAMD_OP_CUBIC_PHASE && D > SFAMD_MAX_CUTOFF_V) return SFAMD_E_CUTOFF;
     if (m0 < 0 || m0 >= n) return SFAMD_E_ARG;
   }
   return 0;
+}
+
+// second-parameter derivative weight of a photon-number-conserving two-mode
+// gate: d/dp1 G[N][n] = i (wa n + wb N) G[N][n]
+void bs_weights(int kind, int* wa, int* wb) {
+  *wa = kind == SFAMD_OP_CROSS_KERR ? 0 : 1;
+  *wb = kind == SFAMD_OP_BEAMSPLITTER ? -1 : 0;
 }
 
 // Routing between element-wise and per-fiber kernels: per-fiber when a kernel
@@ -94,14 +107,32 @@ int bs_pass(const void* in, void* out, int64_t N, int D, int64_t sp, int64_t sq,
 }
 
 int bs_adj_pass(const void* psi, const void* lo, void* li, int64_t N, int D, int64_t sp, int64_t sq, const void* g,
-                const void* dB, int conj, double* pb, int nwg, int zp1, int dtype, hipStream_t s) {
+                const void* dB, int conj, int wa, int wb, double* pb, int nwg, int zp1, int dtype, hipStream_t s) {
   if (use_fib(D, sp < sq ? sp : sq)) {
-    if (mfma_supported(D, dtype)) return pair_bs_adj_mfma(psi, lo, li, N, D, sp, sq, g, dB, conj, pb, nwg, zp1, s);
-    return pair_bs_adj_fib(psi, lo, li, N, D, sp, sq, g, dB, conj, pb, nwg, zp1, dtype, s);
+    if (mfma_supported(D, dtype))
+      return pair_bs_adj_mfma(psi, lo, li, N, D, sp, sq, g, dB, conj, wa, wb, pb, nwg, zp1, s);
+    return pair_bs_adj_fib(psi, lo, li, N, D, sp, sq, g, dB, conj, wa, wb, pb, nwg, zp1, dtype, s);
   }
   if (use_slab(N, D, sp, sq, dtype, 1))
-    return pair_bs_adj_slab(psi, lo, li, N, D, sp, sq, g, dB, conj, pb, nwg, zp1, dtype, s);
-  return pair_bs_adj(psi, lo, li, N, D, sp, sq, g, dB, conj, pb, nwg, zp1, dtype, s);
+    return pair_bs_adj_slab(psi, lo, li, N, D, sp, sq, g, dB, conj, wa, wb, pb, nwg, zp1, dtype, s);
+  return pair_bs_adj(psi, lo, li, N, D, sp, sq, g, dB, conj, wa, wb, pb, nwg, zp1, dtype, s);
+}
+
+// two-mode pass of any two-mode kind (S2 on its own kernels, the others on the
+// beamsplitter family)
+int pair_pass(int kind, const void* in, void* out, int64_t N, int D, int64_t sp, int64_t sq, const void* g, int conj,
+              int dtype, hipStream_t s) {
+  if (kind == SFAMD_OP_TWO_MODE_SQUEEZE) return pair_s2(in, out, N, D, sp, sq, g, conj, dtype, s);
+  return bs_pass(in, out, N, D, sp, sq, g, conj, dtype, s);
+}
+
+int pair_adj_pass(int kind, const void* psi, const void* lo, void* li, int64_t N, int D, int64_t sp, int64_t sq,
+                  const void* g, const void* dG, int conj, double* pb, int nwg, int zp1, int dtype, hipStream_t s) {
+  if (kind == SFAMD_OP_TWO_MODE_SQUEEZE)
+    return pair_s2_adj(psi, lo, li, N, D, sp, sq, g, dG, conj, pb, nwg, zp1, dtype, s);
+  int wa, wb;
+  bs_weights(kind, &wa, &wb);
+  return bs_adj_pass(psi, lo, li, N, D, sp, sq, g, dG, conj, wa, wb, pb, nwg, zp1, dtype, s);
 }
 
 int loss_pass(const void* in, void* out, int64_t N, int D, int64_t sb, int64_t sk, const void* c, int dtype,
@@ -130,7 +161,8 @@ int apply_op(const OpPlan& op, int n, int D, const void* table, const void* in, 
         return pair_diag(in, out, NM, D, bra_stride_dm(n, D, op.m0), ket_stride_dm(n, D, op.m0), g, dtype, s);
       return axis_diag(in, out, NK, D, ket_stride(n, D, op.m0), g, 0, dtype, s);
     case SFAMD_OP_DISPLACEMENT:
-    case SFAMD_OP_SQUEEZE: {
+    case SFAMD_OP_SQUEEZE:
+    case SFAMD_OP_CUBIC_PHASE: {
       const int par = op.kind == SFAMD_OP_SQUEEZE;
       if (op.mixed_in) {
         const int64_t sk = ket_stride_dm(n, D, op.m0);
@@ -142,12 +174,17 @@ int apply_op(const OpPlan& op, int n, int D, const void* table, const void* in, 
       return dense_pass(in, out, NK, D, ket_stride(n, D, op.m0), g, 0, par, dtype, s);
     }
     case SFAMD_OP_BEAMSPLITTER:
+    case SFAMD_OP_TWO_MODE_SQUEEZE:
+    case SFAMD_OP_MZGATE:
+    case SFAMD_OP_CROSS_KERR:
       if (op.mixed_in) {
         if (!tmp) return SFAMD_E_WORKSPACE;
-        RET_IF(bs_pass(in, tmp, NM, D, bra_stride_dm(n, D, op.m0), bra_stride_dm(n, D, op.m1), g, 0, dtype, s));
-        return bs_pass(tmp, out, NM, D, ket_stride_dm(n, D, op.m0), ket_stride_dm(n, D, op.m1), g, 1, dtype, s);
+        RET_IF(pair_pass(op.kind, in, tmp, NM, D, bra_stride_dm(n, D, op.m0), bra_stride_dm(n, D, op.m1), g, 0,
+                         dtype, s));
+        return pair_pass(op.kind, tmp, out, NM, D, ket_stride_dm(n, D, op.m0), ket_stride_dm(n, D, op.m1), g, 1,
+                         dtype, s);
       }
-      return bs_pass(in, out, NK, D, ket_stride(n, D, op.m0), ket_stride(n, D, op.m1), g, 0, dtype, s);
+      return pair_pass(op.kind, in, out, NK, D, ket_stride(n, D, op.m0), ket_stride(n, D, op.m1), g, 0, dtype, s);
     case SFAMD_OP_LOSS: {
       const void* src = in;
       if (op.mix_first) {
@@ -181,7 +218,8 @@ int adjoint_op(const OpPlan& op, int n, int D, const void* table, const void* ps
                              pass0, nwg, 1, dtype, s);
       return axis_diag_adj(psi, lo, li, NK, D, ket_stride(n, D, op.m0), g, kerr, 0, pass0, nwg, 1, dtype, s);
     case SFAMD_OP_DISPLACEMENT:
-    case SFAMD_OP_SQUEEZE: {
+    case SFAMD_OP_SQUEEZE:
+    case SFAMD_OP_CUBIC_PHASE: {
       const int64_t DD = (int64_t)D * D;
       const void* dU0 = cptr(g, DD, dtype);
       const void* dU1 = cptr(g, 2 * DD, dtype);
@@ -195,17 +233,21 @@ int adjoint_op(const OpPlan& op, int n, int D, const void* table, const void* ps
       }
       return dense_adj_pass(psi, lo, li, NK, D, ket_stride(n, D, op.m0), g, dU0, dU1, 0, par, pass0, nwg, 1, dtype, s);
     }
-    case SFAMD_OP_BEAMSPLITTER: {
+    case SFAMD_OP_BEAMSPLITTER:
+    case SFAMD_OP_TWO_MODE_SQUEEZE:
+    case SFAMD_OP_MZGATE:
+    case SFAMD_OP_CROSS_KERR: {
       const void* dB = cptr(g, bs_nb(D), dtype);
+      const int k = op.kind;
       if (op.mixed_in) {
         const int64_t bp = bra_stride_dm(n, D, op.m0), bq = bra_stride_dm(n, D, op.m1);
         const int64_t kp = ket_stride_dm(n, D, op.m0), kq = ket_stride_dm(n, D, op.m1);
-        RET_IF(bs_pass(psi, t1, NM, D, bp, bq, g, 0, dtype, s));
-        RET_IF(bs_adj_pass(t1, lo, t2, NM, D, kp, kq, g, dB, 1, pass1, nwg, 0, dtype, s));
-        return bs_adj_pass(psi, t2, li, NM, D, bp, bq, g, dB, 0, pass0, nwg, 0, dtype, s);
+        RET_IF(pair_pass(k, psi, t1, NM, D, bp, bq, g, 0, dtype, s));
+        RET_IF(pair_adj_pass(k, t1, lo, t2, NM, D, kp, kq, g, dB, 1, pass1, nwg, 0, dtype, s));
+        return pair_adj_pass(k, psi, t2, li, NM, D, bp, bq, g, dB, 0, pass0, nwg, 0, dtype, s);
       }
-      return bs_adj_pass(psi, lo, li, NK, D, ket_stride(n, D, op.m0), ket_stride(n, D, op.m1), g, dB, 0, pass0, nwg,
-                         1, dtype, s);
+      return pair_adj_pass(k, psi, lo, li, NK, D, ket_stride(n, D, op.m0), ket_stride(n, D, op.m1), g, dB, 0, pass0,
+                           nwg, 1, dtype, s);
     }
     case SFAMD_OP_LOSS: {
       const int64_t D3 = (int64_t)D * D * D;
@@ -256,16 +298,16 @@ bool fusable(const OpPlan& op) {
                           op.kind == SFAMD_OP_BEAMSPLITTER);
 }
 
-int op_nmodes(const OpPlan& op) { return op.kind == SFAMD_OP_BEAMSPLITTER ? 2 : 1; }
+int op_nmodes(const OpPlan& op) { return two_mode_kind(op.kind) ? 2 : 1; }
 
 bool op_in(const OpPlan& op, unsigned mask) {
   if (!(mask >> op.m0 & 1u)) return false;
-  return op.kind != SFAMD_OP_BEAMSPLITTER || (mask >> op.m1 & 1u);
+  return !two_mode_kind(op.kind) || (mask >> op.m1 & 1u);
 }
 
 unsigned op_mask(const OpPlan& op) {
   unsigned m = 1u << op.m0;
-  if (op.kind == SFAMD_OP_BEAMSPLITTER) m |= 1u << op.m1;
+  if (two_mode_kind(op.kind)) m |= 1u << op.m1;
   return m;
 }
 
@@ -458,7 +500,9 @@ extern "C" const char* sfamd_status_string(int code) {
 
 extern "C" int64_t sfamd_gate_table_size(int kind, int D) {
   if (D < 1 || D > SFAMD_MAX_CUTOFF) return -1;
-  if (kind == SFAMD_OP_BEAMSPLITTER && D > SFAMD_MAX_CUTOFF_BS) return -1;
+  if (kind < 0 || kind > SFAMD_OP_CUBIC_PHASE) return -1;
+  if (two_mode_kind(kind) && D > SFAMD_MAX_CUTOFF_BS) return -1;
+  if (kind == SFAMD_OP_CUBIC_PHASE && D > SFAMD_MAX_CUTOFF_V) return -1;
   return table_size(kind, D);
 }
 
@@ -480,7 +524,7 @@ extern "C" int sfamd_apply_1mode(int kind, const void* in, void* out, void* tmp,
   CHECK_ARG(in && out && table && in != out && n >= 1);
   CHECK_ARG(dtype == SFAMD_C64 || dtype == SFAMD_C128);
   if (kind != SFAMD_OP_ROTATION && kind != SFAMD_OP_KERR && kind != SFAMD_OP_DISPLACEMENT &&
-      kind != SFAMD_OP_SQUEEZE)
+      kind != SFAMD_OP_SQUEEZE && kind != SFAMD_OP_CUBIC_PHASE)
     return SFAMD_E_KIND;
   RET_IF(validate_op(kind, mode, -1, n, D));
   OpPlan op{kind, mode, -1, mixed ? 1 : 0, 0, 0};
@@ -493,6 +537,16 @@ extern "C" int sfamd_apply_bs(const void* in, void* out, void* tmp, int n, int D
   CHECK_ARG(dtype == SFAMD_C64 || dtype == SFAMD_C128);
   RET_IF(validate_op(SFAMD_OP_BEAMSPLITTER, mode1, mode2, n, D));
   OpPlan op{SFAMD_OP_BEAMSPLITTER, mode1, mode2, mixed ? 1 : 0, 0, 0};
+  return apply_op(op, n, D, table, in, out, tmp, dtype, (hipStream_t)stream);
+}
+
+extern "C" int sfamd_apply_2mode(int kind, const void* in, void* out, void* tmp, int n, int D, int mode1, int mode2,
+                                 int mixed, const void* table, int dtype, void* stream) {
+  CHECK_ARG(in && out && table && in != out && n >= 2);
+  CHECK_ARG(dtype == SFAMD_C64 || dtype == SFAMD_C128);
+  if (!two_mode_kind(kind)) return SFAMD_E_KIND;
+  RET_IF(validate_op(kind, mode1, mode2, n, D));
+  OpPlan op{kind, mode1, mode2, mixed ? 1 : 0, 0, 0};
   return apply_op(op, n, D, table, in, out, tmp, dtype, (hipStream_t)stream);
 }
 

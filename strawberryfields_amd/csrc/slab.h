@@ -12,8 +12,8 @@ bool slab_supported(int64_t N, int D, int64_t S_out, int64_t Sin, int dtype, int
 int pair_bs_slab(const void* in, void* out, int64_t N, int D, int64_t sp, int64_t sq, const void* blocks, int conj,
                  int dtype, hipStream_t s);
 int pair_bs_adj_slab(const void* psi, const void* lam_out, void* lam_in, int64_t N, int D, int64_t sp, int64_t sq,
-                     const void* blocks, const void* dblocks, int conj, double* partial, int nwg, int zp1, int dtype,
-                     hipStream_t s);
+                     const void* blocks, const void* dblocks, int conj, int wa, int wb, double* partial, int nwg,
+                     int zp1, int dtype, hipStream_t s);
 int loss_slab(const void* in, void* out, int64_t N, int D, int64_t sb, int64_t sk, const void* coef, int dtype,
               hipStream_t s);
 int loss_adj_slab(const void* psi, const void* lam_out, void* lam_in, int64_t N, int D, int64_t sb, int64_t sk,

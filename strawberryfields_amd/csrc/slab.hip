@@ -224,8 +224,8 @@ k_bs_slab(const C<T>* __restrict__ in, C<T>* __restrict__ out, Tile t, int p_out
 template <typename T, int D, bool ML>
 __global__ void __launch_bounds__(kSlabBlock)
 k_bs_slab_adj(const C<T>* __restrict__ psi, const C<T>* __restrict__ lo_, C<T>* __restrict__ li, Tile t, int p_outer,
-              const C<T>* __restrict__ blk, const C<T>* __restrict__ dblk, int conj, double* partial, int nwg,
-              int zp1) {
+              const C<T>* __restrict__ blk, const C<T>* __restrict__ dblk, int conj, int wa, int wb, double* partial,
+              int nwg, int zp1) {
   extern __shared__ __align__(16) unsigned char smem[];
   C<T>* shx = (C<T>*)smem;
   C<T>* shl = shx + t.cap;
@@ -264,7 +264,7 @@ k_bs_slab_adj(const C<T>* __restrict__ psi, const C<T>* __restrict__ lo_, C<T>* 
       for (int n = 0; n < D; ++n) {
         if (n >= f.c) continue;
         // lam_in[n] = sum_N conj(B'[N][n]) lam[N]; w0 = sum_N conj(lam[N]) dB'[N][n];
-        // w1 = sum_N conj(lam[N]) i sg (n - N) B'[N][n]
+        // w1 = sum_N conj(lam[N]) i sg (wa n + wb N) B'[N][n]
         C<T> acc = mk<T>(0, 0), w0 = mk<T>(0, 0), w1 = mk<T>(0, 0);
 #pragma unroll
         for (int N = 0; N < D; ++N) {
@@ -278,7 +278,7 @@ k_bs_slab_adj(const C<T>* __restrict__ psi, const C<T>* __restrict__ lo_, C<T>* 
           cmac_conj(acc, b, l[N]);
           cmac_conj(w0, l[N], d);
           const C<T> lb = mk<T>(l[N].re * b.re + l[N].im * b.im, l[N].re * b.im - l[N].im * b.re);
-          const T wt = sg * (T)(n - N);
+          const T wt = sg * (T)(wa * (f.lo + n) + wb * (f.lo + N));
           w1.re += -wt * lb.im;
           w1.im += wt * lb.re;
         }
@@ -454,6 +454,7 @@ struct BsPol {
   const Cf* blk;
   const Cf* dblk;
   int conj, p_outer;
+  int wa, wb;  // second-parameter derivative weight w = wa n + wb N
   // gate table (blocks, and derivative blocks for the adjoint) copied to LDS
   static constexpr int64_t table_bytes(bool adj) { return (adj ? 2 : 1) * bs_nb(D) * (int64_t)sizeof(Cf); }
   __device__ BsPol staged(unsigned char* lds, bool adj) const {
@@ -483,10 +484,11 @@ struct BsPol {
   __device__ Cf dentry0(int s, int n, int i, int k) const {
     return sfamd::mf::op_entry(dblk + bs_block_off(s, D), n, conj ? 1 : 0, i, k);
   }
-  // d/dphi of B'[N][n] = i sg (n - N) B'[N][n], sg = -1 on the conjugated side
+  // second-parameter derivative i sg (wa n + wb N) B'[N][n], sg = -1 on the conjugated side
   __device__ Cf dentry1(int s, int n, int i, int k) const {
     const Cf m = entry(s, n, i, k);
-    const float w = (conj ? -1.f : 1.f) * (float)(k - i);
+    const int lo = bs_lo(s, D);
+    const float w = (conj ? -1.f : 1.f) * (float)(wa * (lo + k) + wb * (lo + i));
     return mk<float>(-w * m.im, w * m.re);
   }
 };
@@ -769,18 +771,20 @@ static int bs_slab_launch(const void* in, void* out, const Tile& t, int p_outer,
 
 template <typename T, int D>
 static int bs_slab_adj_launch(const void* psi, const void* lo, void* li, const Tile& t, int p_outer,
-                              const void* blocks, const void* dblocks, int conj, double* partial, int nwg, int zp1,
-                              hipStream_t s) {
+                              const void* blocks, const void* dblocks, int conj, int wa, int wb, double* partial,
+                              int nwg, int zp1, hipStream_t s) {
   bool ml;
   const size_t lds = slab_lds(t, 2, sizeof(C<T>), 2 * bs_nb(D) * (int64_t)sizeof(C<T>), &ml);
   if (ml) {
     RET_IF(allow_lds(k_bs_slab_adj<T, D, true>, lds));
     hipLaunchKernelGGL((k_bs_slab_adj<T, D, true>), dim3(nwg), dim3(kSlabBlock), lds, s, CP(T, psi), CP(T, lo),
-                       MP(T, li), t, p_outer, CP(T, blocks), CP(T, dblocks), conj, partial, nwg, zp1);
+                       MP(T, li), t, p_outer, CP(T, blocks), CP(T, dblocks), conj, wa, wb, partial, nwg,
+                       zp1);
   } else {
     RET_IF(allow_lds(k_bs_slab_adj<T, D, false>, lds));
     hipLaunchKernelGGL((k_bs_slab_adj<T, D, false>), dim3(nwg), dim3(kSlabBlock), lds, s, CP(T, psi), CP(T, lo),
-                       MP(T, li), t, p_outer, CP(T, blocks), CP(T, dblocks), conj, partial, nwg, zp1);
+                       MP(T, li), t, p_outer, CP(T, blocks), CP(T, dblocks), conj, wa, wb, partial, nwg,
+                       zp1);
   }
   return (int)hipGetLastError();
 }
@@ -845,7 +849,7 @@ static int bs_slab_mfma(const void* in, void* out, int64_t N, int D, int64_t sp,
   switch (D) {
 #define CASE(d)                                                     \
   case d: {                                                         \
-    BsPol<d> pol{(const Cf*)blocks, nullptr, conj, p_outer};        \
+    BsPol<d> pol{(const Cf*)blocks, nullptr, conj, p_outer, 0, 0};  \
     return slab_mfma_launch<d>(in, out, t, pol, s);                 \
   }
     SFAMD_SLAB_D(CASE)
@@ -856,15 +860,15 @@ static int bs_slab_mfma(const void* in, void* out, int64_t N, int D, int64_t sp,
 }
 
 static int bs_slab_mfma_adj(const void* psi, const void* lo, void* li, int64_t N, int D, int64_t sp, int64_t sq,
-                            const void* blocks, const void* dblocks, int conj, double* partial, int nwg, int zp1,
-                            hipStream_t s) {
+                            const void* blocks, const void* dblocks, int conj, int wa, int wb, double* partial,
+                            int nwg, int zp1, hipStream_t s) {
   Tile t;
   const int p_outer = sp > sq;
   if (!plan_tile_mfma(t, N, D, p_outer ? sp : sq, p_outer ? sq : sp, kMfmaTile)) return SFAMD_E_ARG;
   switch (D) {
 #define CASE(d)                                                                    \
   case d: {                                                                        \
-    BsPol<d> pol{(const Cf*)blocks, (const Cf*)dblocks, conj, p_outer};            \
+    BsPol<d> pol{(const Cf*)blocks, (const Cf*)dblocks, conj, p_outer, wa, wb};    \
     return slab_mfma_adj_launch<d>(psi, lo, li, t, pol, partial, nwg, zp1, s);     \
   }
     SFAMD_SLAB_D(CASE)
@@ -927,10 +931,10 @@ int pair_bs_slab(const void* in, void* out, int64_t N, int D, int64_t sp, int64_
 }
 
 int pair_bs_adj_slab(const void* psi, const void* lam_out, void* lam_in, int64_t N, int D, int64_t sp, int64_t sq,
-                     const void* blocks, const void* dblocks, int conj, double* partial, int nwg, int zp1, int dtype,
-                     hipStream_t s) {
+                     const void* blocks, const void* dblocks, int conj, int wa, int wb, double* partial, int nwg,
+                     int zp1, int dtype, hipStream_t s) {
   if (dtype == SFAMD_C64)
-    return bs_slab_mfma_adj(psi, lam_out, lam_in, N, D, sp, sq, blocks, dblocks, conj, partial, nwg, zp1, s);
+    return bs_slab_mfma_adj(psi, lam_out, lam_in, N, D, sp, sq, blocks, dblocks, conj, wa, wb, partial, nwg, zp1, s);
   Tile t;
   const int p_outer = sp > sq;
   if (!plan_tile(t, N, D, p_outer ? sp : sq, p_outer ? sq : sp, slab_limit(dtype, 1))) return SFAMD_E_ARG;
@@ -938,9 +942,9 @@ int pair_bs_adj_slab(const void* psi, const void* lam_out, void* lam_in, int64_t
 #define CASE(d)                                                                                                 \
   case d:                                                                                                       \
     return dtype == SFAMD_C64 ? bs_slab_adj_launch<float, d>(psi, lam_out, lam_in, t, p_outer, blocks, dblocks, \
-                                                             conj, partial, nwg, zp1, s)                       \
+                                                             conj, wa, wb, partial, nwg, zp1, s)               \
                               : bs_slab_adj_launch<double, d>(psi, lam_out, lam_in, t, p_outer, blocks,        \
-                                                              dblocks, conj, partial, nwg, zp1, s);
+                                                              dblocks, conj, wa, wb, partial, nwg, zp1, s);
     SFAMD_SLAB_D(CASE)
 #undef CASE
     default:

@@ -18,6 +18,10 @@ import torch
 _current = []
 
 
+# hbar of the front end (sf.hbar default); only the cubic phase gate uses it
+HBAR = 2.0
+
+
 class RegRef:
     def __init__(self, ind):
         self.ind = ind
@@ -203,6 +207,61 @@ class BSgate(Gate):
     def _apply(self, reg, backend, **kwargs):
         theta, phi = self._vals()
         backend.beamsplitter(theta, phi, *reg)
+
+
+class S2gate(Gate):
+    """Two-mode squeezing (strawberryfields/ops.py:2033-2078)."""
+
+    ns = 2
+
+    def __init__(self, r, phi=0.0):
+        super().__init__([r, phi])
+
+    def _apply(self, reg, backend, **kwargs):
+        r, phi = self._vals()
+        backend.two_mode_squeeze(r, phi, *reg)
+
+
+class MZgate(Gate):
+    """Mach-Zehnder interferometer (strawberryfields/ops.py:1951-2004); as in the
+    reference (ops.py:485-501), the dagger negates the first parameter only and
+    a zero phi_in skips the gate."""
+
+    ns = 2
+
+    def __init__(self, phi_in, phi_ex):
+        super().__init__([phi_in, phi_ex])
+
+    def _apply(self, reg, backend, **kwargs):
+        phi_in, phi_ex = self._vals()
+        backend.mzgate(phi_in, phi_ex, *reg)
+
+
+class CKgate(Gate):
+    """Cross-Kerr interaction exp(i kappa n1 n2) (strawberryfields/ops.py:2219-2257)."""
+
+    ns = 2
+
+    def __init__(self, kappa):
+        super().__init__([kappa])
+
+    def _apply(self, reg, backend, **kwargs):
+        (kappa,) = self._vals()
+        backend.cross_kerr_interaction(kappa, *reg)
+
+
+class Vgate(Gate):
+    """Cubic phase exp(i gamma x^3 / (3 hbar)) (strawberryfields/ops.py:1735-1779);
+    the backend call takes gamma sqrt(hbar / 2)."""
+
+    ns = 1
+
+    def __init__(self, gamma):
+        super().__init__([gamma])
+
+    def _apply(self, reg, backend, **kwargs):
+        (gamma,) = self._vals()
+        backend.cubic_phase(gamma * np.sqrt(HBAR / 2), *reg)
 
 
 class LossChannel(Operation):

@@ -35,6 +35,14 @@ def apply(be, name, modes, p):
         be.beamsplitter(p[0], p[1], modes[0], modes[1])
     elif name == "LOSS":
         be.loss(p[0], modes[0])
+    elif name == "S2":
+        be.two_mode_squeeze(p[0], p[1], modes[0], modes[1])
+    elif name == "MZ":
+        be.mzgate(p[0], p[1], modes[0], modes[1])
+    elif name == "CK":
+        be.cross_kerr_interaction(p[0], modes[0], modes[1])
+    elif name == "V":
+        be.cubic_phase(p[0], modes[0])
     else:
         raise ValueError(name)
 
@@ -525,7 +533,6 @@ def test_per_fiber_kernels_match_oracle_and_fd(dtype, pure, n, D):
     rng = np.random.default_rng(200 + n * D)
     ops = circuits.layered(n, 1, rng, sd=0.3, loss=not pure)
     ref, rpure = fo.run_ops(ops, n, D, pure=pure)
-    be = run_backend(ops, n, D, dtype, pure=pure, fuse=0) if False else None
     cc = runtime.CompiledCircuit([(o[0], o[1]) for o in ops], n, D, not pure, dtype, DEV, fuse=0)
     rows = np.zeros((len(ops), 2))
     for i, (_, _, p) in enumerate(ops):
@@ -542,7 +549,6 @@ def test_per_fiber_kernels_match_oracle_and_fd(dtype, pure, n, D):
     gflat = np.array([g[i, j] for i, (_, _, p) in enumerate(ops) for j in range(len(p))])
     rel = np.linalg.norm(gflat - g_fd) / np.linalg.norm(g_fd)
     assert rel < (1e-8 if dtype == C128 else 1e-5), rel
-    del be, flat
 
 
 # ------------------------------------------------------------------ batched circuits
@@ -700,3 +706,184 @@ def test_c5_structure_dm_layers_with_loss_checkpoints(dtype):
     gflat = np.array([g[i, j] for i, (_, _, p) in enumerate(ops) for j in range(len(p))])
     rel = np.linalg.norm(gflat[r["sel"]] - r["g_fd"]) / np.linalg.norm(r["g_fd"])
     assert rel < (1e-7 if dtype == C128 else 1e-4), rel
+
+
+# ------------------------------------------------------------------ S2, MZ, CK, V
+def _s2_dense_from_blocks(blk, D):
+    """S2 table: block s = D - 1 + d holds G[N][n] for out1 - out2 = in1 - in2 = d."""
+    mat = np.zeros((D,) * 4, dtype=complex)
+    off = 0
+    for s in range(2 * D - 1):
+        lo, hi = max(0, s - D + 1), min(s, D - 1)
+        c, d = hi - lo + 1, s - (D - 1)
+        b = blk[off : off + c * c].reshape(c, c)
+        for a in range(c):
+            for k in range(c):
+                N, n = lo + a, lo + k
+                mat[N, n, N - d, n - d] = b[a, k]
+        off += c * c
+    return mat, off
+
+
+def _fd4(f, x, h=1e-3):
+    return (-f(x + 2 * h) + 8 * f(x + h) - 8 * f(x - h) + f(x - 2 * h)) / (12 * h)
+
+
+@pytest.mark.parametrize("dtype", [C128, C64])
+@pytest.mark.parametrize("D", [1, 2, 6, 15, 20])
+def test_two_mode_gate_tables(dtype, D):
+    """S2, MZ and CK tables against the oracle matrices; the first-parameter
+    derivative blocks against difference quotients of the oracle, and the
+    second-parameter phase rules the adjoint kernels use (dS2/dphi =
+    i (out1 - in1) S2, dMZ/dphi_ex = i in1 MZ) against the same quotients."""
+    # FP64 recurrences on both sides carry up to ~3e-12 each at cutoff 20
+    tol = (1e-13 if D < 15 else 1e-11) if dtype == C128 else 1e-6
+    dtol = (1e-8 if D < 15 else 1e-6) if dtype == C128 else 1e-5
+    i1 = np.arange(D)[None, :, None, None]  # in1 index of mat[out1, in1, out2, in2]
+    o1 = np.arange(D)[:, None, None, None]
+    cases = [
+        ("S2", fo.two_mode_squeezing, _s2_dense_from_blocks, [(0.3, 0.7), (-0.4, 2.1), (0.9, -1.3)]),
+        ("MZ", fo.mzgate, _bs_dense_from_blocks, [(0.3, 0.7), (-2.4, 2.1), (np.pi, -1.3)]),
+        ("CK", lambda k, _p, D: fo.cross_kerr(k, D), _bs_dense_from_blocks, [(0.3, 0.0), (-1.7, 0.0)]),
+    ]
+    for kind, fn, unpack, params in cases:
+        for p0, p1 in params:
+            t = to_np(runtime.build_gate(kind, p0, p1, D, dtype, DEV))
+            G, nb = unpack(t, D)
+            ref = fn(p0, p1, D)
+            assert np.abs(G - ref).max() < tol, (kind, p0, p1)
+            if kind == "CK":  # exact: a difference quotient of e^{i kappa n1 n2} loses ~(n1 n2)^5 h^4
+                d0 = 1j * (o1 * np.arange(D)[None, None, :, None]) * ref
+            else:
+                d0 = _fd4(lambda x: fn(x, p1, D), p0)  # noqa: B023
+            scale = max(1.0, np.abs(d0).max())
+            assert np.abs(unpack(t[nb:], D)[0] - d0).max() < dtol * scale, (kind, "d0")
+            if kind != "CK":
+                d1 = _fd4(lambda x: fn(p0, x, D), p1)  # noqa: B023
+                rule = 1j * ((o1 - i1) if kind == "S2" else i1) * ref
+                assert np.abs(rule - d1).max() < 1e-6 * max(1.0, np.abs(d1).max()), (kind, "d1 rule")
+
+
+@pytest.mark.parametrize("dtype", [C128, C64])
+def test_two_mode_squeezing_tables_match_mpmath(dtype):
+    """S2 tables against the 50-digit closed form of tests/golden/gates_mpmath.npz."""
+    g = np.load(os.path.join(GOLDEN, "gates_mpmath.npz"))
+    keys = [k for k in g.files if k.startswith("S2_") and not k.endswith("_params")]
+    assert len(keys) == 4
+    for key in keys:
+        r, phi, D = g[key + "_params"]
+        t = to_np(runtime.build_gate("S2", r, phi, int(D), dtype, DEV))
+        tol = (1e-13 if D < 15 else 5e-12) if dtype == C128 else 1e-6
+        assert np.abs(_s2_dense_from_blocks(t, int(D))[0] - g[key]).max() < tol, key
+
+
+@pytest.mark.parametrize("dtype", [C128, C64])
+@pytest.mark.parametrize("D", [1, 2, 6, 15, 20, 32])
+def test_cubic_phase_tables(dtype, D):
+    tol = 1e-12 if dtype == C128 else 2e-6
+    for g in [0.3, -1.1, 2.5, 6.0]:
+        t = to_np(runtime.build_gate("V", g, 0.0, D, dtype, DEV))
+        DD = D * D
+        V = fo.cubic_phase(g, D)
+        assert np.abs(t[:DD].reshape(D, D) - V).max() < tol, g
+        dV = _fd4(lambda x: fo.cubic_phase(x, D), g, h=1e-4)
+        assert np.abs(t[DD : 2 * DD].reshape(D, D) - dV).max() < 1e-6 * max(1.0, np.abs(dV).max()), g
+        assert np.abs(t[2 * DD :]).max() == 0.0
+
+
+_NEW_GATE_CASES = [
+    # (pure, n, D): small states (element-wise kernels), inner runs >= 16 on
+    # the outer axes (per-fiber / matrix-core kernels), >= 2^16 states with the
+    # innermost pair (LDS slab tiles)
+    (True, 3, 5),
+    (False, 2, 5),
+    (True, 4, 6),
+    (False, 3, 5),
+    (True, 4, 16),
+    (False, 2, 16),
+]
+
+
+def _new_gate_ops(n, rng, pure):
+    ops = []
+    for m in range(n):
+        ops.append(("D", (m,), [float(rng.uniform(0.1, 0.4)), float(rng.uniform(0, 2 * np.pi))]))
+    pairs = [(0, 1), (n - 1, n - 2)] + ([(n - 2, n - 1), (0, n - 1)] if n > 2 else [])
+    for a, b in pairs:
+        ops.append(("S2", (a, b), [float(rng.uniform(-0.3, 0.3)), float(rng.uniform(0, 2 * np.pi))]))
+        ops.append(("MZ", (a, b), [float(rng.uniform(0.3, 2.8)), float(rng.uniform(0, 2 * np.pi))]))
+        ops.append(("CK", (b, a), [float(rng.uniform(-1, 1))]))
+    for m in range(n):
+        ops.append(("V", (m,), [float(rng.uniform(-0.4, 0.4))]))
+    if not pure:
+        ops.append(("LOSS", (0,), [0.8]))
+        ops.append(("S2", (0, 1), [0.2, 0.3]))
+    return ops
+
+
+@pytest.mark.parametrize("dtype", [C128, C64])
+@pytest.mark.parametrize("pure,n,D", _NEW_GATE_CASES)
+def test_new_gates_match_oracle_and_fd(dtype, pure, n, D):
+    """S2gate, MZgate, CKgate and Vgate on every kernel route: forward through
+    the backend against the oracle; gradients of a photon-number cost against
+    central differences of the complex128 oracle on a seeded parameter subset."""
+    rng = np.random.default_rng(400 + 10 * n + D)
+    ops = _new_gate_ops(n, rng, pure)
+    ref, rpure = fo.run_ops(ops, n, D, pure=pure)
+    be = run_backend(ops, n, D, dtype, pure=pure)
+    st = be.state()
+    got = to_np(st.ket() if rpure else st.dm())
+    assert np.abs(got - ref).max() < TOL[dtype] * 10
+    cc = runtime.CompiledCircuit([(o[0], o[1]) for o in ops], n, D, not pure, dtype, DEV, fuse=0)
+    rows = np.zeros((len(ops), 2))
+    for i, (_, _, p) in enumerate(ops):
+        rows[i, : len(p)] = p
+    params = torch.tensor(rows, dtype=torch.float64, device=DEV, requires_grad=True)
+    st0 = torch.as_tensor(fo.vacuum(n, D, pure=pure), dtype=dtype, device=DEV)
+    out = runtime.CircuitFunction.apply(params, st0, cc, None)
+    assert np.abs(to_np(out) - ref).max() < TOL[dtype] * 10
+    idx = np.indices([D] * n).sum(axis=0)
+    weights = (idx - 1.0) ** 2
+    probs = runtime.all_fock_probs(out, n, D, not rpure).to(torch.float64)
+    (probs * torch.as_tensor(weights, device=DEV)).sum().backward()
+    g = params.grad.cpu().numpy()
+    gflat = np.array([g[i, j] for i, (_, _, p) in enumerate(ops) for j in range(len(p))])
+    nsel = len(gflat) if D ** (n if pure else 2 * n) <= 4096 else 10
+    sel = np.sort(rng.choice(len(gflat), size=min(nsel, len(gflat)), replace=False))
+    g_fd = _fd_subset(ops, n, D, pure, weights, sel)
+    rel = np.linalg.norm(gflat[sel] - g_fd) / np.linalg.norm(g_fd)
+    assert rel < (1e-7 if dtype == C128 else 1e-4), rel
+
+
+def test_new_gates_reference_known_answers():
+    """The reference's own known-answer tests through the HIP backend:
+    two-mode squeezed vacuum on every pair of 4 modes
+    (test_twomode_squeezing_operation.py:44-79), MZ on |1,1>
+    (test_mzgate_operation.py:41-60), cross-Kerr and cubic phase on uniform
+    states (test_nongaussian_gates.py:30-102)."""
+    D = 5
+    k = np.arange(D)
+    for r, p in [(0.1, 0.0), (0.15, 2.0943951023931953)]:
+        amp = np.zeros((D, D), dtype=complex)
+        amp[k, k] = (np.exp(1j * p) * np.tanh(r)) ** k / np.cosh(r)
+        for modes in itertools.combinations(range(4), 2):
+            be = run_backend([("S2", modes, (r, p))], 4, D, C128)
+            red = to_np(be.state().reduced_dm(list(modes)))
+            assert np.abs(red - np.einsum("ac,bd->abcd", amp, amp.conj())).max() < 1e-13, modes
+    one = np.zeros((6, 6), dtype=complex)
+    one[1, 1] = 1.0
+    for pin in [0.3, np.pi / 2, 2.5]:
+        be = run_backend([("MZ", (0, 1), (pin, 0.7))], 2, 6, C128, init=one)
+        pr = to_np(be.state().all_fock_probs())
+        assert abs(pr[1, 1] - np.cos(pin) ** 2) < 1e-13 and abs(pr[0, 2] - 0.5 * np.sin(pin) ** 2) < 1e-13
+    for kappa in [0.5, 2.0]:
+        be = run_backend([("CK", (0, 1), (kappa,))], 2, 6, C128, init=np.ones((6, 6), dtype=complex) / 6)
+        assert np.abs(to_np(be.state().ket()) - np.exp(1j * kappa * np.outer(np.arange(6), np.arange(6))) / 6).max() < 1e-14
+    a = fo.annihilation(8).real
+    x3 = np.linalg.matrix_power(a + a.T, 3)
+    from scipy.linalg import expm
+
+    for gamma in [1.0, 4.0]:
+        ket = np.ones(8, dtype=complex) / np.sqrt(8)
+        be = run_backend([("V", (0,), (gamma,))], 1, 8, C128, init=ket)
+        assert np.abs(to_np(be.state().ket()) - expm(1j * gamma * x3 / 6) @ ket).max() < 1e-12

@@ -50,6 +50,13 @@ def test_gate_table_sizes():
     assert lib.sfamd_gate_table_size(_lib.OP_LOSS, D) == D**3
     assert lib.sfamd_gate_table_size(_lib.OP_BEAMSPLITTER, 41) == -1
     assert lib.sfamd_gate_table_size(_lib.OP_SQUEEZE, 65) == -1
+    for k in (_lib.OP_TWO_MODE_SQUEEZE, _lib.OP_MZGATE, _lib.OP_CROSS_KERR):
+        assert lib.sfamd_gate_table_size(k, D) == 2 * nb
+        assert lib.sfamd_gate_table_size(k, _lib.MAX_CUTOFF_BS) > 0
+        assert lib.sfamd_gate_table_size(k, _lib.MAX_CUTOFF_BS + 1) == -1
+    assert lib.sfamd_gate_table_size(_lib.OP_CUBIC_PHASE, D) == 3 * D * D
+    assert lib.sfamd_gate_table_size(_lib.OP_CUBIC_PHASE, _lib.MAX_CUTOFF_V + 1) == -1
+    assert lib.sfamd_gate_table_size(11, D) == -1
 
 
 def _create(ops, n, D, mixed=0, dtype=_lib.C64, fuse=0):
@@ -89,7 +96,14 @@ def test_program_planning_and_validation():
     assert _create([(_lib.OP_ROTATION, 3, -1)], n, D)[0] == -1
     assert _create([(_lib.OP_ROTATION, 0, -1)], n, 65)[0] == -2
     assert _create([(_lib.OP_BEAMSPLITTER, 0, 1)], n, 41)[0] == -2
-    assert _create([(9, 0, -1)], n, D)[0] == -3
+    assert _create([(11, 0, -1)], n, D)[0] == -3
+    for k in (_lib.OP_TWO_MODE_SQUEEZE, _lib.OP_MZGATE, _lib.OP_CROSS_KERR):
+        assert _create([(k, 0, 2)], n, D)[0] == 0
+        assert _create([(k, 0, -1)], n, D)[0] == -1
+        assert _create([(k, 1, 1)], n, D)[0] == -1
+        assert _create([(k, 0, 1)], n, 41)[0] == -2
+    assert _create([(_lib.OP_CUBIC_PHASE, 2, -1)], n, D)[0] == 0
+    assert _create([(_lib.OP_CUBIC_PHASE, 0, -1)], n, 33)[0] == -2
     assert _create([(_lib.OP_MIX, -1, -1)], n, D, mixed=1)[0] == -4
     assert _create([], n, D, dtype=7)[0] == -1
 
@@ -138,6 +152,30 @@ def test_gate_apply_rules():
     assert be.calls[2] == ("squeeze", 0.1, 0.0, 1)
     assert be.calls[3] == ("beamsplitter", 0.4, 0.1, 1, 0)
     assert be.calls[4] == ("loss", 0.9, 0)
+
+
+def test_two_mode_and_cubic_gate_calls():
+    """S2gate, MZgate, CKgate, Vgate reach the backend calls of
+    strawberryfields/ops.py (Vgate with gamma sqrt(hbar / 2), hbar = 2); the dagger
+    negates the first parameter, and a zero first parameter skips the gate
+    (ops.py:485-501), MZgate(0, phi_ex) included, as in the reference."""
+    prog = frontend.Program(3)
+    with prog.context as q:
+        frontend.S2gate(0.3, 0.2) | (q[2], q[0])
+        frontend.MZgate(0.5, 0.7).H | (q[0], q[1])
+        frontend.MZgate(0.0, 0.7) | (q[0], q[1])
+        frontend.CKgate(0.4) | (q[1], q[2])
+        frontend.Vgate(0.6) | q[1]
+        frontend.S2gate(0.3) | (q[0], q[1])
+    be = RecordingBackend()
+    frontend.Engine(be, backend_options={"cutoff_dim": 4}).run(prog)
+    assert be.calls[1:] == [
+        ("two_mode_squeeze", 0.3, 0.2, 2, 0),
+        ("mzgate", -0.5, 0.7, 0, 1),
+        ("cross_kerr_interaction", 0.4, 1, 2),
+        ("cubic_phase", 0.6, 1),
+        ("two_mode_squeeze", 0.3, 0.0, 0, 1),
+    ]
 
 
 def test_free_parameters_bind_at_run():

@@ -24,18 +24,19 @@ GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 # ------------------------------------------------------------------ matrices
 def test_gate_matrices_match_mpmath_closed_forms():
     g = np.load(os.path.join(GOLDEN, "gates_mpmath.npz"))
-    fns = {"D": fo.displacement, "S": fo.squeezing, "BS": fo.beamsplitter}
+    fns = {"D": fo.displacement, "S": fo.squeezing, "BS": fo.beamsplitter, "S2": fo.two_mode_squeezing}
     n = 0
     for k in g.files:
         if k.endswith("_params"):
             continue
         p0, p1, D = g[k + "_params"]
         m = fns[k.split("_")[0]](p0, p1, int(D))
-        # the FP64 recurrences lose ~1e-12 at cutoff 20 (conditioning, not a slip)
-        tol = 1e-13 if D < 15 else 2e-12
+        # the FP64 recurrences lose ~1e-12 (S2: ~3e-12) at cutoff 20
+        # (conditioning, not a slip)
+        tol = 1e-13 if D < 15 else 5e-12
         assert np.abs(m - g[k]).max() < tol, k
         n += 1
-    assert n == 12
+    assert n == 16
 
 
 @pytest.mark.parametrize("D", [5, 10])
@@ -338,3 +339,156 @@ def test_reduced_dm_of_ket_equals_partial_trace_of_full_dm():
                 traced = [m for m in range(n) if m not in modes]
                 full = fo.partial_trace(fo.dm_of(psi, n, True), n, traced)
                 assert np.abs(fo.reduced_dm(psi, n, True, list(modes)) - full).max() < 1e-12
+
+
+# ------------------------------------------------------------ S2, MZ, CK, V
+def _two_mode_generator_expm(gen_of, D, L):
+    """expm of a two-mode generator on an L x L basis, cut to mat[out1, in1, out2, in2]."""
+    from scipy.linalg import expm
+
+    a = fo.annihilation(L)
+    I = np.eye(L)
+    a1, a2 = np.kron(a, I), np.kron(I, a)
+    U = expm(gen_of(a1, a2)).reshape(L, L, L, L)[:D, :D, :D, :D]
+    return U.transpose(0, 2, 1, 3)
+
+
+@pytest.mark.parametrize("r,phi", [(0.3, 0.7), (-0.45, 2.0), (0.2, -2.9)])
+def test_two_mode_squeezing_matches_generator_exponential(r, phi):
+    """The column recurrence against expm of r (e^{i phi} a1^dag a2^dag - h.c.)
+    on a 48-level basis (leakage past it below 1e-13 at these r)."""
+    D = 8
+    ref = _two_mode_generator_expm(
+        lambda a1, a2: r * (np.exp(1j * phi) * a1.conj().T @ a2.conj().T - np.exp(-1j * phi) * a1 @ a2), D, D + 40
+    )
+    assert np.abs(fo.two_mode_squeezing(r, phi, D) - ref).max() < 1e-12
+
+
+@pytest.mark.parametrize("r", [0.0, 0.1, 0.35])
+@pytest.mark.parametrize("p", [0.0, 2.0943951023931953, 4.1887902047863905])
+def test_two_mode_squeezed_vacuum_known_answer(r, p):
+    """test_twomode_squeezing_operation.py:30-79: amplitude delta_kl
+    (e^{ip} tanh r)^k / cosh r, pure and mixed, on every mode pair of 4 modes."""
+    D = 5
+    amp = np.zeros((D, D), dtype=complex)
+    k = np.arange(D)
+    amp[k, k] = (np.exp(1j * p) * np.tanh(r)) ** k / np.cosh(r)
+    st, pure = fo.run_ops([("S2", (0, 1), (r, p))], 2, D)
+    assert np.abs(st - amp).max() < 1e-14
+    dm, _ = fo.run_ops([("S2", (0, 1), (r, p))], 2, D, pure=False)
+    assert np.abs(dm - np.einsum("ac,bd->abcd", amp, amp.conj())).max() < 1e-14
+    for modes in itertools.combinations(range(4), 2):
+        st, pure = fo.run_ops([("S2", modes, (r, p))], 4, D)
+        red = fo.reduced_dm(st, 4, pure, list(modes))
+        assert np.abs(red - np.einsum("ac,bd->abcd", amp, amp.conj())).max() < 1e-14, modes
+
+
+def test_two_mode_squeezing_selection_rule_and_order():
+    """Only out1 - out2 = in1 - in2 entries are nonzero; the difference-conserving
+    application equals the dense operator for both mode orders, pure and mixed,
+    faithful loop and vectorised."""
+    D, n = 4, 3
+    mat = fo.two_mode_squeezing(0.4, 1.1, D)
+    i, j, k, l = np.indices((D,) * 4)
+    assert np.abs(mat[(i - k) != (j - l)]).max() == 0.0
+    rng = np.random.default_rng(21)
+    psi = rng.normal(size=[D] * n) + 1j * rng.normal(size=[D] * n)
+    for faithful in (True, False):
+        for m1, m2 in itertools.permutations(range(n), 2):
+            c = fo.Circuit(n, D, pure=True, faithful=faithful)
+            c._state = psi.copy()
+            c.two_mode_squeeze(0.4, 1.1, m1, m2)
+            ref = (_bs_full(mat, m1, m2, n, D) @ psi.reshape(-1)).reshape([D] * n)
+            assert np.abs(c._state - ref).max() < 1e-12, (m1, m2)
+            cm = fo.Circuit(n, D, pure=False, faithful=faithful)
+            cm._state = fo.mix(psi, n)
+            cm.two_mode_squeeze(0.4, 1.1, m1, m2)
+            assert np.abs(cm._state - fo.mix(ref, n)).max() < 1e-10, (m1, m2, "mixed")
+
+
+@pytest.mark.parametrize("phi_in", [0.0, np.pi / 2, np.pi, 3 * np.pi / 2])
+@pytest.mark.parametrize("phi_ex", [0.0, np.pi / 2, np.pi, 3 * np.pi / 2])
+def test_mzgate_known_answers(phi_in, phi_ex):
+    """test_mzgate_operation.py:21-89 on |1,1>, cutoff 6: normalised output,
+    p11 = cos^2 phi_in, p02 = sin^2 phi_in / 2, and the gate equals its
+    decomposition R(phi_ex) BS(pi/4, pi/2) R(phi_in) BS(pi/4, pi/2) amplitude by amplitude."""
+    D = 6
+    one = np.zeros((D, D), dtype=complex)
+    one[1, 1] = 1.0
+    st, _ = fo.run_ops([("MZ", (0, 1), (phi_in, phi_ex))], 2, D, state=one)
+    p = np.abs(st) ** 2
+    assert abs(p.sum() - 1) < 1e-13
+    assert abs(p[1, 1] - np.cos(phi_in) ** 2) < 1e-13
+    assert abs(p[0, 2] - 0.5 * np.sin(phi_in) ** 2) < 1e-13
+    dec = [
+        ("R", (0,), (phi_ex,)),
+        ("BS", (0, 1), (np.pi / 4, np.pi / 2)),
+        ("R", (0,), (phi_in,)),
+        ("BS", (0, 1), (np.pi / 4, np.pi / 2)),
+    ]
+    st2, _ = fo.run_ops(dec, 2, D, state=one)
+    assert np.abs(st - st2).max() < 1e-13
+
+
+def test_mzgate_mode_unitary_and_special_cases():
+    """Single-photon block equals the documented mode unitary
+    (strawberryfields/ops.py:1960-1985), U(pi, pi) = I and U(0, 0) = i X; the
+    Fock matrix conserves n1 + n2 and is unitary on every complete block."""
+    D = 5
+    for pin, pex in [(0.3, 1.1), (np.pi / 2, 0.4), (2.0, -0.5)]:
+        M = fo.mzgate(pin, pex, D)
+        e, f = np.exp(1j * pin), np.exp(1j * pex)
+        U = 0.5 * np.array([[(e - 1) * f, 1j * (1 + e)], [1j * (1 + e) * f, 1 - e]])
+        got = np.array([[M[1, 1, 0, 0], M[1, 0, 0, 1]], [M[0, 1, 1, 0], M[0, 0, 1, 1]]])
+        assert np.abs(got - U).max() < 1e-14
+        i, j, k, l = np.indices((D,) * 4)
+        assert np.abs(M[(i + k) != (j + l)]).max() < 1e-15
+        for s in range(D):
+            ks = np.arange(s + 1)
+            blk = M[ks[:, None], ks[None, :], (s - ks)[:, None], (s - ks)[None, :]]
+            assert np.abs(blk @ blk.conj().T - np.eye(s + 1)).max() < 1e-13, s
+    M = fo.mzgate(np.pi, np.pi, D)
+    i, j, k, l = np.indices((D,) * 4)
+    assert np.abs(M - ((i == j) & (k == l))).max() < 1e-14
+    M = fo.mzgate(0.0, 0.0, D)
+    assert abs(M[0, 1, 1, 0] - 1j) < 1e-15 and abs(M[1, 0, 0, 1] - 1j) < 1e-15
+
+
+@pytest.mark.parametrize("kappa", np.linspace(0, 2 * np.pi, 7))
+def test_cross_kerr_known_answer(kappa):
+    """test_nongaussian_gates.py:79-102: uniform product state -> e^{i kappa n1 n2} / D,
+    through the dense two-mode apply_gate_BLAS route; the passive selection-rule
+    route gives the same state (CK conserves n1 and n2)."""
+    D = 6
+    ket = np.ones((D, D), dtype=complex) / D
+    st, _ = fo.run_ops([("CK", (0, 1), (kappa,))], 2, D, state=ket)
+    n = np.arange(D)
+    assert np.abs(st - np.exp(1j * kappa * np.outer(n, n)) / D).max() < 1e-14
+    rng = np.random.default_rng(5)
+    psi = rng.normal(size=[D] * 3) + 1j * rng.normal(size=[D] * 3)
+    for m1, m2 in itertools.permutations(range(3), 2):
+        a, _ = fo.run_ops([("CK", (m1, m2), (kappa,))], 3, D, state=psi)
+        c = fo.Circuit(3, D)
+        c._state = psi.copy()
+        b = c.apply_twomode_gate(fo.cross_kerr(kappa, D), [m1, m2], "BSgate")
+        assert np.abs(a - b).max() < 1e-13
+        dm, _ = fo.run_ops([("CK", (m1, m2), (kappa,))], 3, D, state=fo.mix(psi, 3))
+        assert np.abs(dm - fo.mix(a, 3)).max() < 1e-11
+
+
+@pytest.mark.parametrize("gamma", np.linspace(0, 6, 7))
+def test_cubic_phase_known_answer_and_spectral_form(gamma):
+    """test_nongaussian_gates.py:30-61 (uniform ket through expm(i gamma x^3 / 6),
+    x = a + a^dag) and the same unitary from the eigendecomposition of the real
+    symmetric x^3; hbar scaling: V_hbar(gamma) = V_2(gamma sqrt(hbar / 2))."""
+    D = 10
+    a = fo.annihilation(D).real
+    x3 = np.linalg.matrix_power(a + a.T, 3)
+    w, Q = np.linalg.eigh(x3)
+    spec = (Q * np.exp(1j * gamma * w / 6)) @ Q.T
+    V = fo.cubic_phase(gamma, D)
+    assert np.abs(V - spec).max() < 1e-11
+    ket = np.ones(D) / np.sqrt(D)
+    st, _ = fo.run_ops([("V", (0,), (gamma,))], 1, D, state=ket.astype(complex))
+    assert np.abs(st - spec @ ket).max() < 1e-11
+    assert np.abs(fo.cubic_phase(gamma, D, hbar=1.3) - fo.cubic_phase(gamma * np.sqrt(1.3 / 2), D)).max() < 1e-12
