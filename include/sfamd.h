@@ -129,6 +129,14 @@ int sfamd_apply_2mode(int kind, const void* in, void* out, void* tmp, int n,
                       int D, int mode1, int mode2, int mixed,
                       const void* table, int dtype, void* stream);
 
+/* Any single-mode matrix U[D][D] (row = output index, state dtype) on `mode`;
+ * U rho U^dagger for a DM.  The generic case of Circuit.apply_gate_BLAS
+ * (fockbackend/circuit.py:118-217), used by the homodyne conditional
+ * projection (circuit.py:787-795).  `tmp` as for sfamd_apply_1mode. */
+int sfamd_apply_matrix(const void* in, void* out, void* tmp, int n, int D,
+                       int mode, int mixed, const void* U, int dtype,
+                       void* stream);
+
 /* Loss channel on a DM.  Replaces Circuit._apply_channel with the Kraus set of
  * fockbackend/ops.py:471-490 (fockbackend/circuit.py:65-87). */
 int sfamd_apply_loss(const void* in, void* out, int n, int D, int mode,
@@ -196,6 +204,44 @@ int sfamd_fidelity_loss(const void* target, const void* psi, int64_t n_elems,
  * doubles. */
 int sfamd_number_loss(const void* psi, void* g_psi, int n, int D, double* partial,
                       double* out_dev, int dtype, void* stream);
+
+/* ---- measurements and multi-mode preparation ---------------------------- */
+/* Marginal Fock distribution of the sorted `modes` (k of them): out_dev[D^k]
+ * (double, first listed mode most significant) = sum over the other modes of
+ * |psi|^2 or of the DM diagonal.  Replaces the partial trace + diagonal of
+ * Circuit.measure_fock (fockbackend/circuit.py:671-677).  `partial` holds
+ * sfamd_marginal_partials(n, D, k) doubles. */
+int64_t sfamd_marginal_partials(int n, int D, int k);
+int sfamd_marginal_probs(const void* state, int n, int D, int mixed,
+                         const int* modes, int k, double* partial,
+                         double* out_dev, int dtype, void* stream);
+
+/* out = scale * (|0><x| on each of `modes`) in, outcome x[t] for modes[t]
+ * (any order), both sides of a DM.  Replaces ops.project_reset
+ * (fockbackend/ops.py:179-198) followed by the division by the norm
+ * (circuit.py:689-699); the caller passes scale = 1/norm.  in != out. */
+int sfamd_project_reset(const void* in, void* out, int n, int D, int mixed,
+                        const int* modes, const int* outcome, int k,
+                        double scale, int dtype, void* stream);
+
+/* Homodyne density of a single-mode DM rho[D][D] (bra, ket) rotated by
+ * R(-phi), on the grid q_b = -q_mag + b * 2 q_mag / (num_bins - 1):
+ * out_dev[b] = sqrt(1/(pi hbar)) dq sum_{n,m} rho'_nm h_n(x) h_m(x),
+ * x = q / sqrt(hbar), h_n the normalised Hermite functions (double).
+ * Replaces the pdf of Circuit.measure_homodyne (circuit.py:729-765) and
+ * ops.hermiteVals (ops.py:500-515).  D <= SFAMD_MAX_CUTOFF. */
+int sfamd_homodyne_pdf(const void* rho, int D, double phi, double q_mag,
+                       int num_bins, double hbar, double* out_dev, int dtype,
+                       void* stream);
+
+/* Product state of prepare_multimode (fockbackend/circuit.py:393-473): out =
+ * rest (x) sigma with sigma's axis t on register mode modes[t] (any order) and
+ * `rest` the (n-k)-mode DM of the other modes in increasing order.  DMs
+ * (mixed = 1); for k == n, rest may be NULL and sigma may be a ket
+ * (mixed = 0), which gives the permuted ket. */
+int sfamd_embed_modes(const void* rest, const void* sigma, void* out, int n,
+                      int D, const int* modes, int k, int mixed, int dtype,
+                      void* stream);
 
 /* ---- recorded circuits (forward, adjoint, graphs) ----------------------- */
 typedef struct {

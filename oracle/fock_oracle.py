@@ -318,6 +318,7 @@ class Circuit:
         self._num_modes = num
         self._trunc = trunc
         self._pure = pure
+        self._hbar = 2.0
         self.faithful = faithful
         self.reset(pure)
 
@@ -504,6 +505,147 @@ class Circuit:
         perm = [x for x in range(n) if x != mode] + [mode]
         index_perm = [2 * x + i for x in perm for i in (0, 1)]
         self._state = np.transpose(full, np.argsort(index_perm))
+
+    def prepare_multimode(self, state, modes):
+        """circuit.py:393-473: the listed modes (any order) take `state` (ket or
+        DM, tensor or vector/matrix shape); the others keep their reduced state."""
+        if isinstance(modes, int):
+            modes = [modes]
+        D, n, k = self._trunc, self._num_modes, len(modes)
+        state = np.asarray(state)
+        pure_shape, mixed_shape = (D,) * k, (D,) * (2 * k)
+        if state.shape == (D**k,):
+            state = state.reshape(pure_shape)
+        elif state.shape == (D**k, D**k):
+            state = state.reshape(mixed_shape)
+        if state.shape not in (pure_shape, mixed_shape):
+            raise ValueError("Incorrect shape for state preparation")
+        if len(modes) != len(set(modes)):
+            raise ValueError("The specified modes cannot appear multiple times.")
+        if n == k:
+            self._state = state.astype(DEF_TYPE)
+            self._pure = state.shape == pure_shape
+        else:
+            if self._pure:
+                self._state = mix(self._state, n)
+                self._pure = False
+            if state.shape == pure_shape:
+                state = mix(state, k)
+            self._state = np.tensordot(partial_trace(self._state, n, modes), state, axes=0)
+        if list(modes) != list(range(n - k, n)):
+            perm = [x for x in range(n) if x not in modes] + list(modes)
+            if not self._pure:
+                perm = [2 * x + i for x in perm for i in (0, 1)]
+            self._state = np.transpose(self._state, np.argsort(perm))
+
+    # -- measurements (circuit.py:623-801)
+    def project_reset(self, modes, x, state=None):
+        """|0..0><x| on `modes` (fockbackend/ops.py:179-198)."""
+        state = self._state if state is None else state
+        n = self._num_modes
+        src = tuple(x[modes.index(i)] if i in modes else slice(None) for i in range(n))
+        dst = tuple(0 if i in modes else slice(None) for i in range(n))
+        ret = np.zeros_like(state, dtype=DEF_TYPE)
+        if self._pure:
+            ret[dst] = state[src]
+        else:
+            ret[tuple(dst[i // 2] for i in range(2 * n))] = state[tuple(src[i // 2] for i in range(2 * n))]
+        return ret
+
+    def measure_fock(self, modes, select=None):
+        """circuit.py:623-715; numpy's global generator draws the outcome.  The
+        sampled modes' distribution comes from the state before any
+        post-selection, as there."""
+        if select is not None and np.any(np.array(select) == None):  # noqa: E711
+            raise NotImplementedError("Post-selection lists must only contain numerical values.")
+        n, D = self._num_modes, self._trunc
+        state = mix(self._state, n) if self._pure else self._state
+        if select is not None:
+            if len(select) != len(modes):
+                raise ValueError("select and modes differ in length")
+            measure = [i for i, v in zip(modes, select) if v is None]
+            selected = [i for i, v in zip(modes, select) if v is not None]
+            values = [v for v in select if v is not None]
+            self._state = self.project_reset(selected, values)
+            if self.norm() == 0:
+                raise ZeroDivisionError("Measurement has zero probability.")
+            self._state = self._state / self.norm()
+        else:
+            measure = list(modes)
+        outcome = []
+        if measure:
+            unmeasured = [i for i in range(n) if i not in measure]
+            reduced = partial_trace(state, n, unmeasured)
+            k = len(measure)
+            dist = np.ravel(np.einsum("".join(_IDX[i] * 2 for i in range(k)) + "->" + _IDX[:k], reduced).real)
+            dist = dist * ~np.isclose(dist, 0.0)
+            if sum(dist) != 1:
+                i = np.random.choice(list(range(len(dist))), p=dist / sum(dist))
+            else:
+                i = np.random.choice(list(range(len(dist))), p=dist)
+            permuted = [i // D ** (k - 1 - m) % D for m in range(k)]
+            perm = np.argsort(measure)
+            outcome = [0] * k
+            for j in range(k):
+                outcome[perm[j]] = permuted[j]
+            self._state = self.project_reset(measure, outcome)
+            if self.norm() == 0:
+                raise ZeroDivisionError("Measurement has zero probability.")
+            self._state = self._state / self.norm()
+        if select is not None:
+            outcome = list(select)
+        return np.array([outcome])
+
+    def homodyne_pdf(self, rho, phi, q_mag=10, num_bins=100000):
+        """The density of circuit.py:729-760 with ops.hermiteVals (ops.py:500-515):
+        raw physicists' Hermite polynomials, as there."""
+        from math import factorial
+
+        D, mw = self._trunc, 1 / self._hbar
+        rho = phase(-phi, D) @ rho @ phase(-phi, D).conj().T
+        q = np.linspace(-q_mag, q_mag, num_bins)
+        x = np.sqrt(mw) * q
+        H = [np.ones_like(x), 2 * x]
+        for i in range(2, D):
+            H.append(2 * x * H[i - 1] - 2 * (i - 1) * H[i - 2])
+        acc = np.zeros(num_bins, dtype=DEF_TYPE)
+        for a in range(D):
+            for b in range(D):
+                acc += rho[a, b] * H[a] * H[b] / np.sqrt(float(2**a * factorial(a)) * float(2**b * factorial(b)))
+        return q, (acc * (mw / np.pi) ** 0.5 * np.exp(-mw * q**2) * (q[1] - q[0])).real
+
+    def homodyne_eigenstate(self, x, phi):
+        """R(phi) D(alpha) |truncated infinitely squeezed vacuum>, alpha = x sqrt(1/(2 hbar))
+        (circuit.py:772-785)."""
+        from math import factorial
+
+        D = self._trunc
+        v = np.array(
+            [(-0.5) ** (k // 2) * np.sqrt(factorial(k)) / factorial(k // 2) if k % 2 == 0 else 0.0 for k in range(D)],
+            dtype=DEF_TYPE,
+        )
+        alpha = x * np.sqrt(1 / self._hbar / 2)
+        return phase(phi, D) @ displacement(np.abs(alpha), np.angle(alpha), D) @ v
+
+    def measure_homodyne(self, phi, mode, select=None, **kwargs):
+        """circuit.py:717-801 (hbar = 2)."""
+        n = self._num_modes
+        if select is not None:
+            sample = float(select)
+        else:
+            state = mix(self._state, n) if self._pure else self._state
+            reduced = partial_trace(state, n, [i for i in range(n) if i != mode])
+            q, pdf = self.homodyne_pdf(reduced, phi, kwargs.get("max", 10), kwargs.get("num_bins", 100000))
+            probs = pdf / np.sum(pdf)
+            probs[np.abs(probs) < 1e-10] = 0
+            hist = np.random.multinomial(1, probs)
+            sample = q[list(hist).index(1)]
+        e = self.homodyne_eigenstate(sample, phi)
+        P = np.zeros((self._trunc, self._trunc), dtype=DEF_TYPE)
+        P[0] = e.conj()
+        self._state = self.apply_gate_BLAS(P, [mode])
+        self._state = self._state / self.norm()
+        return np.array([[sample]])
 
     def prepare_mode_fock(self, k, mode):
         st = np.zeros(self._trunc, dtype=DEF_TYPE)

@@ -84,6 +84,7 @@ SIGNATURES = {
     "sfamd_apply_1mode": (_I, [_I, _P, _P, _P, _I, _I, _I, _I, _P, _I, _P]),
     "sfamd_apply_bs": (_I, [_P, _P, _P, _I, _I, _I, _I, _I, _P, _I, _P]),
     "sfamd_apply_2mode": (_I, [_I, _P, _P, _P, _I, _I, _I, _I, _I, _P, _I, _P]),
+    "sfamd_apply_matrix": (_I, [_P, _P, _P, _I, _I, _I, _I, _P, _I, _P]),
     "sfamd_apply_loss": (_I, [_P, _P, _I, _I, _I, _P, _I, _P]),
     "sfamd_mix": (_I, [_P, _P, _I, _I, _I, _P]),
     "sfamd_replace_mode": (_I, [_P, _P, _I, _I, _I, _P, _I, _P]),
@@ -101,6 +102,14 @@ SIGNATURES = {
     "sfamd_overlap": (_I, [_P, _P, _I64, _P, _P, _I, _P]),
     "sfamd_fidelity_loss": (_I, [_P, _P, _I64, _P, _P, _P, _I, _P]),
     "sfamd_number_loss": (_I, [_P, _P, _I, _I, _P, _P, _I, _P]),
+    "sfamd_marginal_partials": (_I64, [_I, _I, _I]),
+    "sfamd_marginal_probs": (_I, [_P, _I, _I, _I, ctypes.POINTER(_I), _I, _P, _P, _I, _P]),
+    "sfamd_project_reset": (
+        _I,
+        [_P, _P, _I, _I, _I, ctypes.POINTER(_I), ctypes.POINTER(_I), _I, _D, _I, _P],
+    ),
+    "sfamd_homodyne_pdf": (_I, [_P, _I, _D, _D, _I, _D, _P, _I, _P]),
+    "sfamd_embed_modes": (_I, [_P, _P, _P, _I, _I, ctypes.POINTER(_I), _I, _I, _I, _P]),
     "sfamd_prog_create": (_I, [ctypes.POINTER(OpDesc), _I, _I, _I, _I, _I, _I, ctypes.POINTER(_P)]),
     "sfamd_prog_info": (_I, [_P, ctypes.POINTER(ProgInfo)]),
     "sfamd_prog_state_elems": (_I64, [_P, _I]),

@@ -17,6 +17,9 @@ Backend options of ``begin_circuit`` / ``reset`` (``fockbackend/backend.py:96-12
 TF backend), ``device`` (default ``"cuda"``) and ``checkpoint_segment``
 (operations per checkpoint segment in the backward; None = automatic).
 """
+import math
+import numbers
+
 import numpy as np
 import torch
 
@@ -281,22 +284,151 @@ class HIPFockBackend(BaseFock):
         self._prepare_array(state, modes, pure=False)
 
     def _prepare_array(self, state, modes, pure):
+        """Circuit.prepare_multimode (fockbackend/circuit.py:393-473): the listed
+        modes (any order) take `state`, the others keep their reduced state, and
+        the result is their product (a DM unless every mode is replaced).  The
+        shape decides ket or DM: [D]*k or [D^k] is a ket, [D]*2k or
+        [D^k, D^k] a DM."""
         rm = self._remap_modes(modes)
         if isinstance(rm, int):
             rm = [rm]
+        if len(rm) != len(set(rm)):
+            raise ValueError("The specified modes cannot appear multiple times.")
         st = torch.as_tensor(np.asarray(state) if not isinstance(state, torch.Tensor) else state)
         k = len(rm)
         D = self._D
-        if k == self._n and rm == list(range(self._n)):
-            self._flush()
-            shape = [D] * (k if pure else 2 * k)
-            self._state = st.to(device=self._device, dtype=self._dtype).reshape(shape).clone()
-            self._pure = pure
+        shape = tuple(st.shape)
+        ket_shapes, dm_shapes = ((D,) * k, (D**k,)), ((D,) * (2 * k), (D**k, D**k))
+        if shape not in ket_shapes + dm_shapes:
+            raise ValueError("Incorrect shape for state preparation")
+        # the shapes coincide only at D = 1; then the method's kind decides
+        is_pure = shape in ket_shapes and (pure or shape not in dm_shapes)
+        st = st.to(device=self._device, dtype=self._dtype).reshape([D] * (k if is_pure else 2 * k))
+        if k == 1:
+            self._prepare(st.reshape([D] if is_pure else [D, D]), rm[0])
             return
-        if k != 1:
-            raise NotImplementedError("preparing a state on a subset of several modes is not supported")
-        st = st.reshape([D] if pure else [D, D])
-        self._prepare(st, rm[0])
+        self._flush()
+        if self._state.requires_grad:
+            raise NotImplementedError("state preparation after differentiable gates is not supported")
+        n = self._n
+        if k == n:
+            if rm != list(range(n)):
+                st = runtime.embed_modes(None, st, n, D, rm, not is_pure)
+            self._state = st.clone() if rm == list(range(n)) else st
+            self._pure = is_pure
+            return
+        sigma = runtime.mix(st, k, D) if is_pure else st
+        rest_modes = [m for m in range(n) if m not in rm]
+        rest = runtime.reduced_dm(self._state, n, D, not self._pure, rest_modes)
+        self._state = runtime.embed_modes(rest, sigma, n, D, rm, True)
+        self._pure = False
+
+    # ------------------------------------------------------------ measurements
+    def _measured_state(self):
+        self._flush()
+        if self._state.requires_grad:
+            raise NotImplementedError("measurement of a state that requires grad is not supported")
+        return self._state
+
+    def _marginal_at(self, st, modes, outcome):
+        """Unnormalised probability of `outcome` on `modes` (any order)."""
+        srt = sorted(modes)
+        p = runtime.marginal_probs(st, self._n, self._D, not self._pure, srt)
+        idx = tuple(int(outcome[list(modes).index(m)]) for m in srt)
+        return float(p[idx].item())
+
+    def measure_fock(self, modes, shots=1, select=None, **kwargs):
+        """Fock measurement with optional post-selection (fockbackend/backend.py:290-295,
+        circuit.py:623-715): the outcome is drawn with numpy's global generator from the
+        marginal distribution of the measured modes (or taken from `select`), the
+        measured modes are projected and reset to vacuum, and the state is
+        renormalised.  As there, `select` must hold an integer for every mode."""
+        if shots != 1:
+            raise NotImplementedError(
+                "{} backend currently does not support shots != 1 for Fock measurement".format(self.short_name)
+            )
+        rm = self._remap_modes(modes)
+        if isinstance(rm, int):
+            rm = [rm]
+        if select is not None and np.any(np.array(select) == None):  # noqa: E711
+            raise NotImplementedError("Post-selection lists must only contain numerical values.")
+        st = self._measured_state()
+        n, D, mixed = self._n, self._D, not self._pure
+        if select is not None:
+            if len(select) != len(rm):
+                raise ValueError(
+                    "When performing post-selection, the number of "
+                    "selected values (including None) must match the number of measured modes"
+                )
+            if not all(isinstance(v, int) for v in select):
+                raise TypeError("The post-select list elements either be integers or None")
+            if any(v < 0 or v >= D for v in select):
+                raise IndexError("post-selected photon numbers must lie in [0, cutoff_dim)")
+            outcome = list(select)
+        else:
+            srt = sorted(rm)
+            raw = np.ravel(runtime.marginal_probs(st, n, D, mixed, srt).cpu().numpy())
+            dist = raw * ~np.isclose(raw, 0.0)
+            if sum(dist) != 1:
+                i = np.random.choice(list(range(len(dist))), p=dist / sum(dist))
+            else:
+                i = np.random.choice(list(range(len(dist))), p=dist)
+            permuted = [i // D ** (len(srt) - 1 - j) % D for j in range(len(srt))]
+            perm = np.argsort(rm)
+            outcome = [0] * len(rm)
+            for j in range(len(rm)):
+                outcome[perm[j]] = permuted[j]
+        # squared norm of the projected state: the outcome's unnormalised probability
+        w = self._marginal_at(st, rm, outcome) if select is not None else float(raw[i])
+        if w == 0:
+            raise ZeroDivisionError("Measurement has zero probability.")
+        self._state = runtime.project_reset(st, n, D, mixed, rm, outcome, 1 / (w if mixed else np.sqrt(w)))
+        return np.array([outcome])
+
+    def measure_homodyne(self, phi, mode, shots=1, select=None, **kwargs):
+        """Homodyne measurement of x cos(phi) + p sin(phi) (fockbackend/backend.py:184-198,
+        circuit.py:717-801), hbar = 2: the density of the mode's reduced DM on the
+        linspace(-max, max, num_bins) grid (GPU), a numpy multinomial draw, then the
+        projection |0><x_phi| (x_phi the truncated, displaced and rotated infinitely
+        squeezed vacuum) and renormalisation.
+        Keyword args: ``max`` (default 10), ``num_bins`` (default 100000)."""
+        if shots != 1:
+            raise NotImplementedError(
+                "{} backend currently does not support shots != 1 for homodyne measurement".format(self.short_name)
+            )
+        m = self._remap_modes(mode)
+        st = self._measured_state()
+        n, D, mixed = self._n, self._D, not self._pure
+        hbar = 2.0
+        rho = runtime.reduced_dm(st, n, D, mixed, [m])
+        if select is not None:
+            if not isinstance(select, numbers.Number):
+                raise TypeError("Selected measurement result must be of numeric type.")
+            sample = float(select)
+        else:
+            q_mag = kwargs.get("max", 10)
+            num_bins = kwargs.get("num_bins", 100000)
+            probs = runtime.homodyne_pdf(rho, D, phi, q_mag, num_bins, hbar).cpu().numpy()
+            q = np.linspace(-q_mag, q_mag, num_bins)
+            probs = probs / np.sum(probs)
+            probs[np.abs(probs) < 1e-10] = 0
+            hist = np.random.multinomial(1, probs)
+            sample = q[list(hist).index(1)]
+        alpha = sample * np.sqrt(1 / hbar / 2)
+        k = np.arange(D)
+        vac = np.array(
+            [(-0.5) ** (j // 2) * np.sqrt(math.factorial(j)) / math.factorial(j // 2) if j % 2 == 0 else 0.0 for j in k]
+        )
+        Dm = runtime.gate_matrix("D", abs(alpha), np.angle(alpha), D, self._dtype, self._device)
+        rot = torch.as_tensor(np.exp(1j * phi * k), dtype=self._dtype, device=self._device)
+        e = rot * (Dm @ torch.as_tensor(vac, dtype=self._dtype, device=self._device))
+        w = float(torch.vdot(e, rho.reshape(D, D) @ e).real.item())
+        if w <= 0:
+            raise ZeroDivisionError("Measurement has zero probability.")
+        P = torch.zeros((D, D), dtype=self._dtype, device=self._device)
+        P[0] = e.conj() / np.sqrt(w)
+        self._state = runtime.apply_matrix(st, n, D, mixed, m, P)
+        return np.array([[sample]])
 
     # ------------------------------------------------------------ queries
     def is_vacuum(self, tol=0.0, **kwargs):
@@ -339,4 +471,4 @@ class HIPFockBackend(BaseFock):
     def _not_here(self, *args, **kwargs):
         raise NotImplementedError("not implemented by the HIP Fock backend")
 
-    thermal_loss = measure_homodyne = measure_fock = measure_threshold = prepare_gkp = _not_here
+    thermal_loss = measure_threshold = prepare_gkp = _not_here

@@ -124,13 +124,36 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return r;  // valid in thread 0
 }
 
+// ---- mode subsets ----------------------------------------------------------
+// kept: the listed modes in increasing order; traced: the others, increasing
+struct ModeList {
+  int32_t n, k;
+  int32_t kept[32];
+  int32_t traced[32];
+};
+
+inline int make_modelist(int n, const int* modes, int k, ModeList* ml) {
+  if (n > 32 || k < 0 || k > n) return SFAMD_E_ARG;
+  ml->n = n;
+  ml->k = k;
+  bool used[32] = {false};
+  for (int t = 0; t < k; ++t) {
+    if (modes[t] < 0 || modes[t] >= n || used[modes[t]]) return SFAMD_E_ARG;
+    if (t > 0 && modes[t] <= modes[t - 1]) return SFAMD_E_ARG;
+    used[modes[t]] = true;
+    ml->kept[t] = modes[t];
+  }
+  int u = 0;
+  for (int m = 0; m < n; ++m)
+    if (!used[m]) ml->traced[u++] = m;
+  return 0;
+}
+
 // ---- internal launchers (implemented in the .hip files) ------------------
 namespace sfamd {
 
 int64_t table_size(int kind, int D);
 
-// gate-table builder: ops[i] has kind, table offset (complex elements) and the
-// two parameters read from params[2*pidx + slot]
 // one operation of a table build: kind and table offset; the operation's
 // parameters are params[2*(first + i) + slot]
 struct BuildDesc {

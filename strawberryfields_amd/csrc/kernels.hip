@@ -549,11 +549,6 @@ k_reduce_grads(const double* __restrict__ partial, double* __restrict__ grads, s
 }
 
 // ------------------------------------------------------------------ readout
-struct ModeList {
-  int32_t n, k;
-  int32_t kept[32];
-  int32_t traced[32];
-};
 
 // DM element index of the diagonal entry (I, I)
 __device__ __forceinline__ int64_t diag_index(int64_t I, int n, int D) {
@@ -1096,23 +1091,6 @@ extern "C" int sfamd_all_fock_probs_vjp(const void* state, const void* g_probs, 
     hipLaunchKernelGGL(k_probs_vjp<double>, dim3(grid_for(N)), dim3(kBlock), 0, s, CP(double, state),
                        (const double*)g_probs, MP(double, g_state), n, D, mixed, N);
   return (int)hipGetLastError();
-}
-
-static int make_modelist(int n, const int* modes, int k, ModeList* ml) {
-  if (n > 32 || k < 0 || k > n) return SFAMD_E_ARG;
-  ml->n = n;
-  ml->k = k;
-  bool used[32] = {false};
-  for (int t = 0; t < k; ++t) {
-    if (modes[t] < 0 || modes[t] >= n || used[modes[t]]) return SFAMD_E_ARG;
-    if (t > 0 && modes[t] <= modes[t - 1]) return SFAMD_E_ARG;
-    used[modes[t]] = true;
-    ml->kept[t] = modes[t];
-  }
-  int u = 0;
-  for (int m = 0; m < n; ++m)
-    if (!used[m]) ml->traced[u++] = m;
-  return 0;
 }
 
 extern "C" int64_t sfamd_reduced_dm_partials(int n, int D, int k, int mixed) {

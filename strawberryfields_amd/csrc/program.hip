@@ -531,6 +531,16 @@ extern "C" int sfamd_apply_1mode(int kind, const void* in, void* out, void* tmp,
   return apply_op(op, n, D, table, in, out, tmp, dtype, (hipStream_t)stream);
 }
 
+extern "C" int sfamd_apply_matrix(const void* in, void* out, void* tmp, int n, int D, int mode, int mixed,
+                                  const void* U, int dtype, void* stream) {
+  CHECK_ARG(in && out && U && in != out && n >= 1);
+  CHECK_ARG(dtype == SFAMD_C64 || dtype == SFAMD_C128);
+  RET_IF(validate_op(SFAMD_OP_DISPLACEMENT, mode, -1, n, D));
+  // the dense single-mode path reads only the leading D x D matrix of a table
+  OpPlan op{SFAMD_OP_DISPLACEMENT, mode, -1, mixed ? 1 : 0, 0, 0};
+  return apply_op(op, n, D, U, in, out, tmp, dtype, (hipStream_t)stream);
+}
+
 extern "C" int sfamd_apply_bs(const void* in, void* out, void* tmp, int n, int D, int mode1, int mode2, int mixed,
                               const void* table, int dtype, void* stream) {
   CHECK_ARG(in && out && table && in != out && n >= 2);

@@ -4,9 +4,10 @@ Same names and behaviour as the reference for the operations this backend
 runs: ``Program`` (``strawberryfields/program.py``), ``Engine`` /
 ``LocalEngine`` (``engine.py:48-572``), ``Result`` (``result.py``), free
 parameters (``parameters.py``: ``prog.params`` / ``eng.run(args=...)``) and the
-operations ``Sgate``, ``Dgate``, ``Rgate``, ``Kgate``, ``BSgate``,
-``LossChannel``, ``Vacuum``, ``Fock``, ``Coherent``, ``Squeezed``
-(``ops.py:541-1948``).  ``Gate.apply`` keeps the reference rules: a gate whose
+operations ``Sgate``, ``Dgate``, ``Rgate``, ``Kgate``, ``BSgate``, ``S2gate``,
+``MZgate``, ``CKgate``, ``Vgate``, ``LossChannel``, ``Vacuum``, ``Fock``,
+``Coherent``, ``Squeezed``, ``Ket``, ``DensityMatrix``, ``MeasureFock``,
+``MeasureHomodyne`` / ``MeasureX`` / ``MeasureP`` (``ops.py:541-2257``).  ``Gate.apply`` keeps the reference rules: a gate whose
 first parameter is zero is skipped and a daggered gate negates its first
 parameter (``ops.py:485-509``).  With Strawberry Fields installed, use
 ``sf.Engine(backend=HIPFockBackend())`` instead; this module exists so the
@@ -314,12 +315,100 @@ class Squeezed(Operation):
         backend.prepare_squeezed_state(par_evaluate(self.p[0]), par_evaluate(self.p[1]), *reg)
 
 
-class Result:
-    """Outcome of Engine.run (result.py): the final state."""
+class Ket(Operation):
+    """Prepare a ket on one or more modes (ops.py:957-1000)."""
 
     def __init__(self, state):
+        super().__init__([state])
+
+    def _apply(self, reg, backend, **kwargs):
+        backend.prepare_ket_state(self.p[0], reg)
+
+
+class DensityMatrix(Operation):
+    """Prepare a density matrix on one or more modes (ops.py:1003-1050)."""
+
+    def __init__(self, state):
+        super().__init__([state])
+
+    def _apply(self, reg, backend, **kwargs):
+        backend.prepare_dm_state(self.p[0], reg)
+
+
+class Measurement(Operation):
+    """Measurement base (ops.py:262-330): the backend's values are stored on the
+    register references (``reg.val``) and returned to the Engine."""
+
+    def __init__(self, par, select=None):
+        super().__init__(par)
+        self.select = select
+
+    def apply(self, reg, backend, **kwargs):
+        if kwargs.get("shots", 1) is None:
+            return None
+        values = self._apply([r.ind for r in reg], backend, **kwargs)
+        for v, r in zip(np.transpose(values), reg):
+            r.val = v
+        return values
+
+
+class MeasureFock(Measurement):
+    """Photon-counting measurement (ops.py:1147-1190), optional post-selection
+    and Poisson dark counts."""
+
+    def __init__(self, select=None, dark_counts=None):
+        if dark_counts and select:
+            raise NotImplementedError("Post-selection cannot be used together with dark counts.")
+        if dark_counts is not None and not isinstance(dark_counts, (list, tuple)):
+            dark_counts = [dark_counts]
+        if select is not None and not isinstance(select, (list, tuple)):
+            select = [select]
+        self.dark_counts = dark_counts
+        super().__init__([], select)
+
+    def _apply(self, reg, backend, shots=1, **kwargs):
+        samples = np.array(backend.measure_fock(reg, shots=shots, select=self.select, **kwargs))
+        if self.dark_counts is not None:
+            if len(self.dark_counts) != len(reg):
+                raise ValueError(
+                    "The number of dark counts must be equal to the number of measured modes: "
+                    "{} != {}".format(len(self.dark_counts), len(reg))
+                )
+            samples += np.random.poisson(self.dark_counts, samples.shape)
+        return samples
+
+
+class MeasureHomodyne(Measurement):
+    """Homodyne measurement of x cos(phi) + p sin(phi) (ops.py:1230-1283); the
+    backend call is in hbar = 2 units, so values scale by sqrt(hbar / 2)."""
+
+    ns = 1
+
+    def __init__(self, phi, select=None):
+        super().__init__([phi], select)
+
+    def _apply(self, reg, backend, shots=1, **kwargs):
+        s = np.sqrt(HBAR / 2)
+        select = self.select
+        if select is not None:
+            select = select / s
+        phi = par_evaluate(self.p[0])
+        return s * backend.measure_homodyne(phi, *reg, shots=shots, select=select, **kwargs)
+
+
+MeasureX = MeasureHomodyne(0)
+MeasureP = MeasureHomodyne(np.pi / 2)
+
+
+class Result:
+    """Outcome of Engine.run (result.py): the final state and the samples of
+    the last measurement of each mode, ``samples`` shaped (shots, measured
+    modes in increasing order), ``samples_dict`` mode -> list of values."""
+
+    def __init__(self, state, samples=None, samples_dict=None):
         self.state = state
-        self.samples = np.empty((0, 0))
+        self.samples = np.empty((0, 0)) if samples is None else samples
+        self.samples_dict = {} if samples_dict is None else samples_dict
 
 
 class LocalEngine:
@@ -347,11 +436,20 @@ class LocalEngine:
             self._initialized = True
         if args:
             program.bind_params(args)
+        samples_dict = {}
         for cmd in program.circuit:
-            cmd.op.apply(cmd.reg, self.backend)
+            val = cmd.op.apply(cmd.reg, self.backend)
+            if val is not None:  # engine.py:427-442
+                for i, r in enumerate(cmd.reg):
+                    samples_dict.setdefault(r.ind, []).append(np.asarray(val)[:, i])
         self.run_progs.append(program)
         modes = kwargs.get("modes", None)
-        return Result(self.backend.state(modes=modes))
+        samples = np.empty((0, 0))
+        if samples_dict:  # engine.py:473-497: last value of each mode, sorted by mode
+            last = {key: v[-1] for key, v in samples_dict.items()}
+            samples = np.transpose([v for _, v in sorted(last.items())])
+        samples_dict = {k: [np.array(i) for i in v] for k, v in samples_dict.items()}
+        return Result(self.backend.state(modes=modes), samples, samples_dict)
 
 
 Engine = LocalEngine

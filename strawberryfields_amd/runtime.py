@@ -426,6 +426,111 @@ def replace_mode(dm, n, D, mode, sigma):
     return out
 
 
+def marginal_probs(state, n, D, mixed, modes):
+    """Fock distribution of the sorted `modes` (float64 device tensor [D]*k)."""
+    lib = _lib.load()
+    k = len(modes)
+    dev = state.device
+    part = _scratch(lib.sfamd_marginal_partials(n, D, k), torch.float64, dev)
+    out = torch.empty([D] * k, dtype=torch.float64, device=dev)
+    arr = (ctypes.c_int * max(1, k))(*modes)
+    _lib.call(
+        "sfamd_marginal_probs",
+        _lib.ptr(state.contiguous()),
+        n,
+        D,
+        int(mixed),
+        arr,
+        k,
+        _lib.ptr(part),
+        _lib.ptr(out),
+        _lib.dtype_code(state.dtype),
+        _lib.stream_ptr(dev),
+    )
+    return out
+
+
+def project_reset(state, n, D, mixed, modes, outcome, scale):
+    """scale * |0><outcome| on `modes` (both sides of a DM)."""
+    k = len(modes)
+    out = torch.empty_like(state)
+    _lib.call(
+        "sfamd_project_reset",
+        _lib.ptr(state.contiguous()),
+        _lib.ptr(out),
+        n,
+        D,
+        int(mixed),
+        (ctypes.c_int * max(1, k))(*[int(m) for m in modes]),
+        (ctypes.c_int * max(1, k))(*[int(x) for x in outcome]),
+        k,
+        float(scale),
+        _lib.dtype_code(state.dtype),
+        _lib.stream_ptr(state.device),
+    )
+    return out
+
+
+def homodyne_pdf(rho, D, phi, q_mag, num_bins, hbar):
+    """Homodyne density of a single-mode DM on the linspace(-q_mag, q_mag, num_bins)
+    grid (float64 device tensor, sums to ~1 for a state inside the cutoff)."""
+    out = torch.empty(num_bins, dtype=torch.float64, device=rho.device)
+    _lib.call(
+        "sfamd_homodyne_pdf",
+        _lib.ptr(rho.contiguous()),
+        D,
+        float(phi),
+        float(q_mag),
+        int(num_bins),
+        float(hbar),
+        _lib.ptr(out),
+        _lib.dtype_code(rho.dtype),
+        _lib.stream_ptr(rho.device),
+    )
+    return out
+
+
+def apply_matrix(state, n, D, mixed, mode, U):
+    """Single-mode matrix U [D, D] on `mode` (U rho U^dagger on a DM)."""
+    out = torch.empty_like(state)
+    tmp = torch.empty_like(state) if mixed else None
+    _lib.call(
+        "sfamd_apply_matrix",
+        _lib.ptr(state.contiguous()),
+        _lib.ptr(out),
+        _lib.ptr(tmp) if tmp is not None else None,
+        n,
+        D,
+        mode,
+        int(mixed),
+        _lib.ptr(U.to(state.dtype).contiguous()),
+        _lib.dtype_code(state.dtype),
+        _lib.stream_ptr(state.device),
+    )
+    return out
+
+
+def embed_modes(rest, sigma, n, D, modes, mixed):
+    """rest (x) sigma with sigma's axes on `modes` (any order), see sfamd_embed_modes."""
+    dev = sigma.device
+    out = torch.empty([D] * (2 * n if mixed else n), dtype=sigma.dtype, device=dev)
+    k = len(modes)
+    _lib.call(
+        "sfamd_embed_modes",
+        _lib.ptr(rest.contiguous()) if rest is not None else None,
+        _lib.ptr(sigma.contiguous()),
+        _lib.ptr(out),
+        n,
+        D,
+        (ctypes.c_int * k)(*[int(m) for m in modes]),
+        k,
+        int(mixed),
+        _lib.dtype_code(sigma.dtype),
+        _lib.stream_ptr(dev),
+    )
+    return out
+
+
 def build_gate(kind, p0, p1, D, dtype, device):
     """Gate-table entry of one gate (matrix + derivatives) as a flat device tensor."""
     lib = _lib.load()

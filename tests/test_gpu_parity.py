@@ -887,3 +887,184 @@ def test_new_gates_reference_known_answers():
         ket = np.ones(8, dtype=complex) / np.sqrt(8)
         be = run_backend([("V", (0,), (gamma,))], 1, 8, C128, init=ket)
         assert np.abs(to_np(be.state().ket()) - expm(1j * gamma * x3 / 6) @ ket).max() < 1e-12
+
+
+# ------------------------------------------------------------------ measurements, preparations
+def _load(be, state, pure):
+    n = state.ndim if pure else state.ndim // 2
+    be.begin_circuit(n, cutoff_dim=state.shape[0], pure=pure, dtype=be._dtype, device=DEV)
+    be._state = torch.as_tensor(state, dtype=be._dtype, device=DEV).clone()
+    be._pure = pure
+
+
+def _backend_with(state, pure, dtype):
+    be = HIPFockBackend()
+    be._dtype = dtype
+    _load(be, state, pure)
+    return be
+
+
+@pytest.mark.parametrize("dtype", [C128, C64])
+@pytest.mark.parametrize("pure", [True, False])
+def test_marginal_projection_and_matrix_kernels(dtype, pure):
+    """Marginal distributions on every mode subset, |0><x| projections and a
+    generic single-mode matrix against the oracle."""
+    n, D = 3, 5
+    st = rand_state(n, D, pure, 77)
+    t = torch.as_tensor(st, dtype=dtype, device=DEV)
+    probs = fo.all_fock_probs(st, n, pure)
+    for k in range(0, n + 1):
+        for modes in itertools.combinations(range(n), k):
+            got = to_np(runtime.marginal_probs(t, n, D, not pure, list(modes)))
+            ref = probs.sum(axis=tuple(m for m in range(n) if m not in modes))
+            assert np.abs(got - ref).max() < (1e-14 if dtype == C128 else 1e-6), modes
+    c = fo.Circuit(n, D, pure=pure)
+    for modes, x in [([1], [2]), ([2, 0], [1, 3]), ([0, 1, 2], [4, 0, 1])]:
+        got = to_np(runtime.project_reset(t, n, D, not pure, modes, x, 0.5))
+        ref = 0.5 * c.project_reset(modes, x, state=st)
+        assert np.abs(got - ref).max() < TOL[dtype], modes
+    rng = np.random.default_rng(3)
+    U = rng.normal(size=(D, D)) + 1j * rng.normal(size=(D, D))
+    for mode in range(n):
+        got = to_np(runtime.apply_matrix(t, n, D, not pure, mode, torch.as_tensor(U, device=DEV)))
+        ref = c.apply_gate_BLAS(U, [mode], state=st)
+        assert np.abs(got - ref).max() < TOL[dtype] * 10, mode
+
+
+@pytest.mark.parametrize("pure", [True, False])
+def test_measure_fock_matches_oracle_with_the_same_seed(pure):
+    """Same numpy seed, same outcome and conditional state as the oracle, for
+    sampled and post-selected measurements; a None in `select` is rejected by
+    both (circuit.py:628-629)."""
+    n, D = 3, 5
+    st = rand_state(n, D, pure, 78)
+    for modes, select in [([2, 1], [None, 3]), ([0, 2], [1, None])]:
+        with pytest.raises(NotImplementedError):
+            _backend_with(st, pure, C128).measure_fock(modes, select=select)
+        with pytest.raises(NotImplementedError):
+            fo.Circuit(n, D, pure=pure).measure_fock(modes, select=select)
+    cases = [([0], None), ([2, 0], None), ([0, 1, 2], None), ([1], [2]), ([2, 0], [3, 1])]
+    for seed, (modes, select) in enumerate(cases):
+        be = _backend_with(st, pure, C128)
+        c = fo.Circuit(n, D, pure=pure)
+        c._state = st.copy()
+        np.random.seed(100 + seed)
+        got = be.measure_fock(modes, select=select)
+        np.random.seed(100 + seed)
+        ref = c.measure_fock(modes, select=select)
+        assert got.tolist() == ref.tolist(), (modes, select)
+        assert np.abs(to_np(be._state) - c._state).max() < 1e-12, (modes, select)
+
+
+def test_measure_fock_reference_known_answers():
+    """test_fock_measurement.py:107-150 and test_postselection.py:52-68 through the
+    backend: Fock products give their photon numbers and the post-selected
+    beamsplitter output conserves photon number."""
+    D = 5
+    np.random.seed(4)
+    for pure in (True, False):
+        for idx, modes in enumerate([[0], [1, 2], [2, 0, 1]]):
+            ns = [idx % D, (idx + 2) % D, (idx + 4) % D]
+            be = HIPFockBackend()
+            be.begin_circuit(3, cutoff_dim=D, pure=pure, dtype=C128, device=DEV)
+            for m, k in enumerate(ns):
+                be.prepare_fock_state(k, m)
+            assert be.measure_fock(modes).tolist() == [[ns[m] for m in modes]]
+            assert abs(be.state().trace() - 1) < 1e-12
+        be = HIPFockBackend()
+        be.begin_circuit(2, cutoff_dim=D, pure=pure, dtype=C128, device=DEV)
+        for nphot in range(D - 1):
+            be.reset(pure=pure)
+            be.prepare_fock_state(nphot, 0)
+            be.prepare_fock_state(D - 1 - nphot, 1)
+            be.beamsplitter(np.pi / 4, 0, 0, 1)
+            try:
+                r1 = be.measure_fock([0], select=[D // 2])
+            except ZeroDivisionError:
+                continue
+            r2 = be.measure_fock([1])
+            assert r1[0][0] + r2[0][0] == D - 1
+
+
+@pytest.mark.parametrize("dtype", [C128, C64])
+def test_homodyne_density_matches_oracle(dtype):
+    D = 12
+    rng = np.random.default_rng(9)
+    A = rng.normal(size=(D, D)) + 1j * rng.normal(size=(D, D))
+    rho = A @ A.conj().T
+    rho /= np.trace(rho)
+    c = fo.Circuit(1, D)
+    for phi in [0.0, 0.8, -2.2]:
+        q, ref = c.homodyne_pdf(rho, phi, 10, 20001)
+        got = to_np(runtime.homodyne_pdf(torch.as_tensor(rho, dtype=dtype, device=DEV), D, phi, 10, 20001, 2.0))
+        assert np.abs(got - ref).max() < (1e-15 if dtype == C128 else 1e-9), phi
+        assert abs(got.sum() - 1) < 1e-6
+
+
+@pytest.mark.parametrize("pure", [True, False])
+def test_measure_homodyne_matches_oracle_with_the_same_seed(pure):
+    n, D = 2, 8
+    st, _ = fo.run_ops([("S", (0,), (0.4, 0.3)), ("D", (1,), (0.5, 1.0)), ("BS", (0, 1), (0.6, 0.2))], n, D, pure=pure)
+    for seed, (phi, mode, select) in enumerate([(0.0, 0, None), (0.7, 1, None), (1.1, 0, 0.35)]):
+        be = _backend_with(st, pure, C128)
+        c = fo.Circuit(n, D, pure=pure)
+        c._state = st.copy()
+        np.random.seed(200 + seed)
+        got = be.measure_homodyne(phi, mode, select=select, num_bins=20001)
+        np.random.seed(200 + seed)
+        ref = c.measure_homodyne(phi, mode, select=select, num_bins=20001)
+        assert got.tolist() == ref.tolist(), (phi, mode)
+        assert np.abs(to_np(be._state) - c._state).max() < 1e-11, (phi, mode)
+    be = HIPFockBackend()
+    be.begin_circuit(1, cutoff_dim=D, pure=pure, dtype=C128, device=DEV)
+    be.squeeze(np.arcsinh(1.0), 0.0, 0)
+    be.displacement(0.8, 0.0, 0)
+    be.measure_homodyne(0, 0, num_bins=4001)
+    assert be.is_vacuum(1e-12)  # test_homodyne.py:27-38
+
+
+@pytest.mark.parametrize("dtype", [C128, C64])
+@pytest.mark.parametrize("pure", [True, False])
+def test_multimode_preparation_matches_oracle(dtype, pure):
+    """prepare_ket_state / prepare_dm_state on mode subsets in any order, and on
+    all modes permuted, against the oracle's prepare_multimode."""
+    n, D = 4, 4
+    base, _ = fo.run_ops([("D", (m,), (0.3, 0.5 * m)) for m in range(n)] + [("BS", (0, 2), (0.5, 0.1))], n, D, pure=pure)
+    rng = np.random.default_rng(10)
+    ket2 = rng.normal(size=(D, D)) + 1j * rng.normal(size=(D, D))
+    ket2 /= np.linalg.norm(ket2)
+    A = rng.normal(size=(D * D, D * D)) + 1j * rng.normal(size=(D * D, D * D))
+    dm2 = (A @ A.conj().T / np.trace(A @ A.conj().T)).reshape(D, D, D, D).transpose(0, 2, 1, 3)
+    full = rng.normal(size=[D] * n) + 1j * rng.normal(size=[D] * n)
+    for state, modes, ket in [
+        (ket2, [3, 1], True),
+        (ket2.reshape(-1), [0, 2], True),
+        (dm2, [1, 2], False),
+        (dm2.reshape(D * D, D * D), [2, 0], False),
+        (full, [2, 0, 3, 1], True),
+    ]:
+        be = _backend_with(base, pure, dtype)
+        c = fo.Circuit(n, D, pure=pure)
+        c._state = base.copy()
+        if ket:
+            be.prepare_ket_state(state, modes)
+        else:
+            be.prepare_dm_state(state, modes)
+        c.prepare_multimode(state, modes)
+        assert be._pure == c._pure, modes
+        assert np.abs(to_np(be._state) - c._state).max() < TOL[dtype] * 10, modes
+
+
+def test_engine_collects_measurement_samples():
+    prog = frontend.Program(3)
+    with prog.context as q:
+        frontend.Fock(2) | q[0]
+        frontend.Fock(1) | q[2]
+        frontend.MeasureFock() | (q[2], q[0])
+        frontend.MeasureX | q[1]
+    be = HIPFockBackend()
+    np.random.seed(1)
+    res = frontend.Engine(be, backend_options={"cutoff_dim": 5, "pure": True}).run(prog)
+    assert res.samples.shape == (1, 3)
+    assert res.samples[0][0] == 2 and res.samples[0][2] == 1
+    assert q[0].val == 2 and q[2].val == 1

@@ -115,6 +115,31 @@ def test_null_pointer_arguments_rejected_without_gpu_work():
     assert lib.sfamd_graph_begin(None) == -1
 
 
+def test_measurement_entry_points_validate_on_the_host():
+    """Bad arguments of the measurement / preparation entry points are
+    rejected before any launch (dummy pointers are never dereferenced)."""
+    lib = _lib.load()
+    a, b = ctypes.c_void_p(16), ctypes.c_void_p(32)
+    I = ctypes.c_int
+    assert lib.sfamd_marginal_partials(3, 5, 2) == 25
+    assert lib.sfamd_marginal_partials(3, 5, 4) == -1
+    # unsorted modes, mode out of range
+    assert lib.sfamd_marginal_probs(a, 3, 5, 0, (I * 2)(2, 0), 2, b, b, 0, None) == -1
+    assert lib.sfamd_marginal_probs(a, 3, 5, 0, (I * 1)(3), 1, b, b, 0, None) == -1
+    # in == out, outcome outside the cutoff, repeated mode
+    assert lib.sfamd_project_reset(a, a, 3, 5, 0, (I * 1)(0), (I * 1)(1), 1, 1.0, 0, None) == -1
+    assert lib.sfamd_project_reset(a, b, 3, 5, 0, (I * 1)(0), (I * 1)(5), 1, 1.0, 0, None) == -1
+    assert lib.sfamd_project_reset(a, b, 3, 5, 1, (I * 2)(1, 1), (I * 2)(0, 0), 2, 1.0, 0, None) == -1
+    # repeated mode, ket embedding of a strict subset, missing rest
+    assert lib.sfamd_embed_modes(a, a, b, 3, 4, (I * 2)(1, 1), 2, 1, 0, None) == -1
+    assert lib.sfamd_embed_modes(a, a, b, 3, 4, (I * 2)(0, 1), 2, 0, 0, None) == -1
+    assert lib.sfamd_embed_modes(None, a, b, 3, 4, (I * 2)(0, 1), 2, 1, 0, None) == -1
+    # cutoff above the homodyne limit, zero bins
+    assert lib.sfamd_homodyne_pdf(a, 65, 0.0, 10.0, 100, 2.0, b, 0, None) == -1
+    assert lib.sfamd_homodyne_pdf(a, 5, 0.0, 10.0, 0, 2.0, b, 0, None) == -1
+    assert lib.sfamd_apply_matrix(a, a, None, 2, 4, 0, 0, b, 0, None) == -1
+
+
 # ------------------------------------------------------------------ front end
 class RecordingBackend:
     short_name = "rec"

@@ -492,3 +492,161 @@ def test_cubic_phase_known_answer_and_spectral_form(gamma):
     st, _ = fo.run_ops([("V", (0,), (gamma,))], 1, D, state=ket.astype(complex))
     assert np.abs(st - spec @ ket).max() < 1e-11
     assert np.abs(fo.cubic_phase(gamma, D, hbar=1.3) - fo.cubic_phase(gamma * np.sqrt(1.3 / 2), D)).max() < 1e-12
+
+
+# ------------------------------------------------------------ measurements, preparations
+def _fock_product(ns, D, pure):
+    c = fo.Circuit(len(ns), D, pure=pure)
+    for m, k in enumerate(ns):
+        c.prepare_mode_fock(k, m)
+    return c
+
+
+@pytest.mark.parametrize("pure", [True, False])
+def test_fock_measurement_known_answers(pure):
+    """test_fock_measurement.py:86-150: Fock product states give their photon
+    numbers on every mode subset, and conditional states stay normalised;
+    test_fock_measurement.py:169-177: the vacuum measures zero."""
+    D = 5
+    np.random.seed(3)
+    choices = [(0,), (1,), (2,), (0, 1), (0, 2), (1, 2), (0, 1, 2), (2, 0)]
+    for idx in range(16):
+        ns = [idx % D, (idx + 1) % D, (idx + 2) % D]
+        modes = list(choices[idx % len(choices)])
+        c = _fock_product(ns, D, pure)
+        res = c.measure_fock(modes)
+        assert res.tolist() == [[ns[m] for m in modes]]
+        assert abs(fo.trace(c._state, 3, c._pure) - 1) < 1e-13
+        # the measured modes are reset to vacuum
+        probs = fo.all_fock_probs(c._state, 3, c._pure)
+        expect = np.zeros([D] * 3)
+        expect[tuple(0 if m in modes else ns[m] for m in range(3))] = 1
+        assert np.abs(probs - expect).max() < 1e-13
+    c = fo.Circuit(3, D, pure=pure)
+    assert c.measure_fock([0, 1, 2]).tolist() == [[0, 0, 0]]
+
+
+@pytest.mark.parametrize("pure", [True, False])
+def test_fock_measurement_tmsv_and_postselection(pure):
+    """test_fock_measurement.py:151-167 (two-mode squeezed vacuum: equal counts)
+    and test_postselection.py:52-68 (post-selecting one beamsplitter output
+    conserves the total photon number in the other)."""
+    D = 6
+    np.random.seed(5)
+    for _ in range(10):
+        c = fo.Circuit(2, D, pure=pure)
+        c.squeeze(-0.25, 0, 0)
+        c.squeeze(0.25, 0, 1)
+        c.beamsplitter(np.pi / 4, np.pi, 0, 1)
+        r = c.measure_fock([0, 1])
+        assert r[0][0] == r[0][1]
+    for nphot in range(D - 1):
+        c = fo.Circuit(2, D, pure=pure)
+        c.prepare_mode_fock(nphot, 0)
+        c.prepare_mode_fock(D - 1 - nphot, 1)
+        c.beamsplitter(np.pi / 4, 0, 0, 1)
+        try:
+            r1 = c.measure_fock([0], select=[D // 2])
+        except ZeroDivisionError:
+            continue
+        r2 = c.measure_fock([1])
+        assert r1[0][0] + r2[0][0] == D - 1
+
+
+def test_fock_measurement_distribution_matches_marginal():
+    """Many draws from one state follow its marginal distribution (chi-square
+    style bound on the largest deviation)."""
+    D = 4
+    st, _ = fo.run_ops([("D", (0,), (0.6, 0.2)), ("BS", (0, 1), (0.7, 0.3))], 2, D)
+    p = fo.all_fock_probs(st, 2, True).sum(axis=1)
+    np.random.seed(11)
+    counts = np.zeros(D)
+    N = 4000
+    for _ in range(N):
+        c = fo.Circuit(2, D)
+        c._state = st.copy()
+        counts[c.measure_fock([0])[0][0]] += 1
+    assert np.abs(counts / N - p / p.sum()).max() < 5 * np.sqrt(0.25 / N)
+
+
+def test_homodyne_density_closed_forms():
+    """pdf of the vacuum is N(0, hbar/2) and of |1> is q^2 e^{-q^2/hbar} (2/hbar)
+    / sqrt(pi hbar) (hbar = 2), times dq; a rotation by phi of a coherent state
+    moves the mean to 2 |alpha| cos(arg alpha - phi)."""
+    D = 12
+    c = fo.Circuit(1, D)
+    vac = np.zeros((D, D), dtype=complex)
+    vac[0, 0] = 1
+    q, pdf = c.homodyne_pdf(vac, 0.0, 10, 20001)
+    dq = q[1] - q[0]
+    assert np.abs(pdf - np.exp(-(q**2) / 2) / np.sqrt(2 * np.pi) * dq).max() < 1e-15
+    one = np.zeros((D, D), dtype=complex)
+    one[1, 1] = 1
+    q, pdf = c.homodyne_pdf(one, 0.7, 10, 20001)
+    assert np.abs(pdf - q**2 * np.exp(-(q**2) / 2) / np.sqrt(2 * np.pi) * dq).max() < 1e-15
+    st, _ = fo.run_ops([("D", (0,), (0.8, 0.4))], 1, 20)
+    c20 = fo.Circuit(1, 20)
+    for phi in [0.0, 0.9]:
+        q, pdf = c20.homodyne_pdf(np.outer(st, st.conj()), phi, 10, 20001)
+        assert abs((q * pdf).sum() - 2 * 0.8 * np.cos(0.4 - phi)) < 1e-9
+
+
+def test_homodyne_known_answers():
+    """test_homodyne.py:27-38 (the measured mode is reset to vacuum),
+    test_postselection.py:20-33 (post-selection returns the selected value) and
+    test_homodyne.py:40-55 (vacuum: mean 0, std 1 over many draws)."""
+    D = 10
+    c = fo.Circuit(1, D)
+    c.squeeze(np.arcsinh(1.0), 0.0, 0)
+    c.displacement(0.8, 0.0, 0)
+    np.random.seed(2)
+    c.measure_homodyne(0, 0, num_bins=4001)
+    assert c.is_vacuum(1e-12)
+    c2 = fo.Circuit(2, D)
+    c2.squeeze(1.0, 0, 0)
+    c2.squeeze(-1.0, 0, 1)
+    c2.beamsplitter(np.pi / 4, 0, 0, 1)
+    assert c2.measure_homodyne(0, 0, select=0.2)[0][0] == 0.2
+    assert abs(fo.trace(c2._state, 2, c2._pure) - 1) < 1e-12
+    red = fo.reduced_dm(c2._state, 2, c2._pure, [0])
+    assert abs(red[0, 0] - 1) < 1e-12
+    xs = []
+    np.random.seed(7)
+    for _ in range(300):
+        v = fo.Circuit(1, D)
+        xs.append(v.measure_homodyne(0, 0, num_bins=4001)[0][0])
+    xs = np.array(xs)
+    std10 = 10 / np.sqrt(len(xs))
+    assert abs(xs.mean()) < std10 and abs(xs.std() - 1) < std10
+
+
+def test_multimode_preparation_known_answers():
+    """test_multimode_state_preparations.py:34-222: product kets and DMs on mode
+    pairs in any order equal successive single-mode preparations; the vector /
+    matrix shapes equal the tensor shapes; other modes keep their state."""
+    rng = np.random.default_rng(8)
+    D = 4
+    k0 = rng.normal(size=D) + 1j * rng.normal(size=D)
+    k1 = rng.normal(size=D) + 1j * rng.normal(size=D)
+    k0, k1 = k0 / np.linalg.norm(k0), k1 / np.linalg.norm(k1)
+    ket = np.outer(k0, k1)
+    rho = np.einsum("ij,kl->ikjl", ket, ket.conj())
+    for pure in (True, False):
+        c = fo.Circuit(4, D, pure=pure)
+        c.prepare_multimode(ket, [3, 1])
+        red = fo.reduced_dm(c._state, 4, c._pure, [1, 3])  # modes in increasing order
+        assert np.abs(red - rho.transpose(2, 3, 0, 1)).max() < 1e-14
+        single = fo.Circuit(4, D, pure=pure)
+        single.prepare(k0, 0)
+        single.prepare(k1, 1)
+        multi = fo.Circuit(4, D, pure=pure)
+        multi.prepare_multimode(ket.reshape(-1), [0, 1])
+        assert np.abs(single._state - multi._state).max() < 1e-14
+        dmat = fo.Circuit(4, D, pure=pure)
+        dmat.prepare_multimode(rho.reshape(D * D, D * D), [0, 1])
+        assert np.abs(dmat._state - multi._state).max() < 1e-14
+    # all modes at once, permuted: a ket stays a ket
+    full = rng.normal(size=[D] * 3) + 1j * rng.normal(size=[D] * 3)
+    c = fo.Circuit(3, D)
+    c.prepare_multimode(full, [2, 0, 1])
+    assert c._pure and np.abs(c._state - full.transpose(1, 2, 0)).max() == 0.0
