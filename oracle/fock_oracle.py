@@ -730,6 +730,37 @@ def mean_photon(state, n, pure, mode):
     return mean, float((k**2 * p).sum() - mean**2)
 
 
+def passive_fock_action(U, psi, n, D):
+    """Exact action of the passive interferometer U (a_j^dag -> sum_i U_ij a_i^dag)
+    on a ket whose support has fewer than D photons in total, by expanding
+    prod_j (b_j^dag)^{k_j} / sqrt(k_j!) |0> photon by photon.  Independent of
+    any gate decomposition."""
+    from math import factorial
+
+    U = np.asarray(U)
+    out = np.zeros([D] * n, dtype=DEF_TYPE)
+    for occ_in in product(*([range(D)] * n)):
+        amp = psi[occ_in]
+        if amp == 0:
+            continue
+        if sum(occ_in) >= D:
+            raise ValueError("input support must hold fewer than D photons")
+        terms = {(0,) * n: amp / np.sqrt(float(np.prod([factorial(k) for k in occ_in])))}
+        for j, k in enumerate(occ_in):
+            for _ in range(k):
+                nxt = {}
+                for occ, a in terms.items():
+                    for i in range(n):
+                        o = list(occ)
+                        o[i] += 1
+                        o = tuple(o)
+                        nxt[o] = nxt.get(o, 0) + a * U[i, j] * np.sqrt(o[i])
+                terms = nxt
+        for occ, a in terms.items():
+            out[occ] += a
+    return out
+
+
 # ----------------------------------------------------------------- circuits
 def run_ops(ops, n, D, pure=True, faithful=False, state=None):
     """Apply a list of (name, modes, params) ops to vacuum (or `state`).

@@ -7,7 +7,8 @@ parameters (``parameters.py``: ``prog.params`` / ``eng.run(args=...)``) and the
 operations ``Sgate``, ``Dgate``, ``Rgate``, ``Kgate``, ``BSgate``, ``S2gate``,
 ``MZgate``, ``CKgate``, ``Vgate``, ``LossChannel``, ``Vacuum``, ``Fock``,
 ``Coherent``, ``Squeezed``, ``Ket``, ``DensityMatrix``, ``MeasureFock``,
-``MeasureHomodyne`` / ``MeasureX`` / ``MeasureP`` (``ops.py:541-2257``).  ``Gate.apply`` keeps the reference rules: a gate whose
+``MeasureHomodyne`` / ``MeasureX`` / ``MeasureP``, ``Interferometer``
+(``ops.py:541-2718``).  ``Gate.apply`` keeps the reference rules: a gate whose
 first parameter is zero is skipped and a daggered gate negates its first
 parameter (``ops.py:485-509``).  With Strawberry Fields installed, use
 ``sf.Engine(backend=HIPFockBackend())`` instead; this module exists so the
@@ -313,6 +314,34 @@ class Squeezed(Operation):
 
     def _apply(self, reg, backend, **kwargs):
         backend.prepare_squeezed_state(par_evaluate(self.p[0]), par_evaluate(self.p[1]), *reg)
+
+
+class Interferometer(Operation):
+    """Passive linear interferometer U (ops.py:2556-2718), compiled into Rgate and
+    BSgate by the rectangular (Clements) mesh, as the fock compiler does."""
+
+    def __init__(self, U, mesh="rectangular", drop_identity=True, tol=1e-6):
+        U = np.asarray(U)
+        super().__init__([U])
+        self.ns = U.shape[0]
+        if mesh != "rectangular":
+            raise NotImplementedError("only the rectangular mesh is implemented")
+        self.mesh, self.drop_identity, self.tol = mesh, drop_identity, tol
+
+    def decompose(self, reg):
+        """[(gate, registers)] in application order."""
+        from .interferometer import interferometer_gates
+
+        out = []
+        for name, modes, p in interferometer_gates(self.p[0], self.drop_identity, self.tol):
+            op = Rgate(p[0]) if name == "R" else BSgate(p[0], p[1])
+            out.append((op, [reg[m] for m in modes]))
+        return out
+
+    def apply(self, reg, backend, **kwargs):
+        regs = [r if isinstance(r, RegRef) else RegRef(r) for r in reg]
+        for op, rr in self.decompose(regs):
+            op.apply(rr, backend, **kwargs)
 
 
 class Ket(Operation):

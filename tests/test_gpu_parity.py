@@ -1068,3 +1068,35 @@ def test_engine_collects_measurement_samples():
     assert res.samples.shape == (1, 3)
     assert res.samples[0][0] == 2 and res.samples[0][2] == 1
     assert q[0].val == 2 and q[2].val == 1
+
+
+@pytest.mark.parametrize("dtype", [C128, C64])
+@pytest.mark.parametrize("pure", [True, False])
+def test_interferometer_matches_exact_passive_action(dtype, pure):
+    """Interferometer(U) (rectangular mesh on the BS / R kernels) on a 4-mode
+    state with fewer than D photons equals the exact creation-operator action
+    of U, for the register in order and permuted."""
+    from scipy.stats import unitary_group
+
+    n, D = 4, 5
+    rng = np.random.default_rng(12)
+    psi = np.zeros([D] * n, dtype=complex)
+    for occ in itertools.product(range(D), repeat=n):
+        if sum(occ) < D:
+            psi[occ] = rng.normal() + 1j * rng.normal()
+    psi /= np.linalg.norm(psi)
+    U = unitary_group.rvs(n, random_state=5)
+    for order in ([0, 1, 2, 3], [2, 0, 3, 1]):
+        prog = frontend.Program(n)
+        with prog.context as q:
+            frontend.Ket(psi if pure else fo.mix(psi, n)) | q
+            frontend.Interferometer(U) | tuple(q[m] for m in order)
+        be = HIPFockBackend()
+        res = frontend.Engine(be, backend_options={"cutoff_dim": D, "pure": pure, "dtype": dtype}).run(prog)
+        # U acts on the register in `order`: permute U onto the mode axes
+        P = np.zeros((n, n))
+        P[order, range(n)] = 1
+        ref = fo.passive_fock_action(P @ U @ P.T, psi, n, D)
+        got = to_np(res.state.ket() if pure else res.state.dm())
+        want = ref if pure else fo.mix(ref, n)
+        assert np.abs(got - want).max() < TOL[dtype] * 10, order

@@ -293,3 +293,57 @@ def test_stage_planner_leaves_mixed_ops_unfused():
     # every op from the first loss on is its own step
     tail = [st for st in steps if st[0] >= first_loss]
     assert all(len(st) == 1 for st in tail)
+
+
+# ------------------------------------------------------------------ interferometer
+def _mode_unitary(gates, N):
+    G = np.identity(N, dtype=complex)
+    for name, modes, p in gates:
+        M = np.identity(N, dtype=complex)
+        if name == "R":
+            M[modes[0], modes[0]] = np.exp(1j * p[0])
+        else:
+            m, n = modes
+            c, s = np.cos(p[0]), np.sin(p[0])
+            M[m, m], M[m, n], M[n, m], M[n, n] = c, -np.exp(-1j * p[1]) * s, np.exp(1j * p[1]) * s, c
+        G = M @ G
+    return G
+
+
+def test_rectangular_mesh_reconstructs_the_unitary():
+    """The Rgate / BSgate list of the rectangular mesh multiplies back to U
+    (BS mode unitary [[cos, -e^{-i phi} sin], [e^{i phi} sin, cos]]) for Haar
+    unitaries, permutations (the zero and swap branches) and diagonal phases;
+    N (N - 1) / 2 beamsplitters; the identity compiles to nothing."""
+    from scipy.stats import unitary_group
+    from strawberryfields_amd.interferometer import interferometer_gates
+
+    for N in [2, 3, 4, 5, 8]:
+        U = unitary_group.rvs(N, random_state=10 + N)
+        g = interferometer_gates(U)
+        assert np.abs(_mode_unitary(g, N) - U).max() < 1e-12, N
+        assert sum(1 for x in g if x[0] == "BS") == N * (N - 1) // 2
+        assert np.abs(_mode_unitary(interferometer_gates(U, drop_identity=False), N) - U).max() < 1e-12
+    for U in [np.eye(4)[[2, 0, 3, 1]], np.diag(np.exp(1j * np.array([0.3, -1.2, 2.0]))), np.eye(3)[[1, 0, 2]] * 1j]:
+        assert np.abs(_mode_unitary(interferometer_gates(U), len(U)) - U).max() < 1e-12
+    assert interferometer_gates(np.eye(5)) == []
+    with pytest.raises(ValueError):
+        interferometer_gates(np.ones((3, 3)))
+
+
+def test_interferometer_frontend_issues_backend_calls():
+    from scipy.stats import unitary_group
+
+    U = unitary_group.rvs(3, random_state=4)
+    prog = frontend.Program(3)
+    with prog.context as q:
+        frontend.Interferometer(U) | (q[2], q[0], q[1])
+    be = RecordingBackend()
+    frontend.Engine(be, backend_options={"cutoff_dim": 3}).run(prog)
+    calls = be.calls[1:]
+    assert calls and all(c[0] in ("rotation", "beamsplitter") for c in calls)
+    # register order is respected: the mesh acts on (q2, q0, q1)
+    regs = {c[-1] for c in calls if c[0] == "rotation"}
+    assert regs <= {0, 1, 2}
+    with pytest.raises(NotImplementedError):
+        frontend.Interferometer(U, mesh="triangular")

@@ -650,3 +650,41 @@ def test_multimode_preparation_known_answers():
     c = fo.Circuit(3, D)
     c.prepare_multimode(full, [2, 0, 1])
     assert c._pure and np.abs(c._state - full.transpose(1, 2, 0)).max() == 0.0
+
+
+def test_passive_fock_action_known_answers():
+    """Single photons follow the columns of U; Hong-Ou-Mandel for a 50:50
+    splitter; the two-mode action equals the (mpmath-pinned) beamsplitter
+    matrix; a 3-mode unitary equals its Clements gate list run through the
+    oracle's R and BS gates."""
+    from scipy.stats import unitary_group
+    from strawberryfields_amd.interferometer import interferometer_gates
+
+    D = 4
+    U = unitary_group.rvs(3, random_state=2)
+    for j in range(3):
+        psi = np.zeros([D] * 3, dtype=complex)
+        psi[tuple(1 if m == j else 0 for m in range(3))] = 1
+        out = fo.passive_fock_action(U, psi, 3, D)
+        for i in range(3):
+            assert abs(out[tuple(1 if m == i else 0 for m in range(3))] - U[i, j]) < 1e-15
+    bs = np.array([[1, -1], [1, 1]]) / np.sqrt(2)
+    psi = np.zeros((D, D), dtype=complex)
+    psi[1, 1] = 1
+    out = fo.passive_fock_action(bs, psi, 2, D)
+    assert abs(out[1, 1]) < 1e-15 and abs(abs(out[2, 0]) ** 2 - 0.5) < 1e-15
+    th, ph = 0.7, 0.3
+    U2 = np.array([[np.cos(th), -np.exp(-1j * ph) * np.sin(th)], [np.exp(1j * ph) * np.sin(th), np.cos(th)]])
+    rng = np.random.default_rng(1)
+    psi = np.zeros((D, D), dtype=complex)
+    for a in range(D):
+        for b in range(D - a):
+            psi[a, b] = rng.normal() + 1j * rng.normal()
+    st, _ = fo.run_ops([("BS", (0, 1), (th, ph))], 2, D, state=psi)
+    assert np.abs(fo.passive_fock_action(U2, psi, 2, D) - st).max() < 1e-14
+    psi3 = np.zeros([D] * 3, dtype=complex)
+    for occ in itertools.product(range(D), repeat=3):
+        if sum(occ) < D:
+            psi3[occ] = rng.normal() + 1j * rng.normal()
+    st, _ = fo.run_ops(interferometer_gates(U), 3, D, state=psi3)
+    assert np.abs(fo.passive_fock_action(U, psi3, 3, D) - st).max() < 1e-13
