@@ -17,8 +17,10 @@
 // slab is larger than the LDS budget.  In LDS the tile is dense in the order
 // (slab, outer axis, mid, run).
 //
-// Gate coefficients (beamsplitter blocks, loss coefficients) are staged in LDS
-// as well when they fit, since lanes of one wave work on different blocks.
+// The element-wise (VALU) kernels stage the gate coefficients (beamsplitter
+// blocks, loss coefficients) in LDS as well when they fit, since lanes of one
+// wave work on different blocks; the matrix-core kernels build each block's
+// operator once per tile from the table in global memory.
 #include "common.h"
 #include "mfma_ops.h"
 #include "slab.h"
@@ -455,19 +457,6 @@ struct BsPol {
   const Cf* dblk;
   int conj, p_outer;
   int wa, wb;  // second-parameter derivative weight w = wa n + wb N
-  // gate table (blocks, and derivative blocks for the adjoint) copied to LDS
-  static constexpr int64_t table_bytes(bool adj) { return (adj ? 2 : 1) * bs_nb(D) * (int64_t)sizeof(Cf); }
-  __device__ BsPol staged(unsigned char* lds, bool adj) const {
-    BsPol p = *this;
-    Cf* m = (Cf*)lds;
-    stage(m, blk, bs_nb(D));
-    p.blk = m;
-    if (adj) {
-      stage(m + bs_nb(D), dblk, bs_nb(D));
-      p.dblk = m + bs_nb(D);
-    }
-    return p;
-  }
   __device__ int size(int s) const { return bs_hi(s, D) - bs_lo(s, D) + 1; }
   __device__ int per_block(const Tile& t, const View& v) const { return v.gs * v.mt * t.Sin; }
   __device__ void fiber(int s, int phi, const Tile& t, const View& v, int& pos0, int& step) const {
@@ -499,18 +488,6 @@ struct LossPol {
   static constexpr bool kTwoParams = false;
   const float* coef;
   const float* dcoef;
-  static constexpr int64_t table_bytes(bool adj) { return (adj ? 2 : 1) * (int64_t)D * D * D * sizeof(float); }
-  __device__ LossPol staged(unsigned char* lds, bool adj) const {
-    LossPol p = *this;
-    float* m = (float*)lds;
-    stage(m, coef, (int64_t)D * D * D);
-    p.coef = m;
-    if (adj) {
-      stage(m + D * D * D, dcoef, (int64_t)D * D * D);
-      p.dcoef = m + D * D * D;
-    }
-    return p;
-  }
   __device__ int size(int di) const {
     const int d = di - (D - 1);
     return D - (d < 0 ? -d : d);
@@ -565,7 +542,7 @@ template <int D, class P>
 __global__ void __launch_bounds__(256) k_slab_mfma(const Cf* __restrict__ in, Cf* __restrict__ out, Tile t, P gpol) {
   extern __shared__ __align__(16) unsigned char smem[];
   Cf* sh = (Cf*)smem;
-  const P pol = gpol.staged((unsigned char*)(sh + t.cap), false);  // synced by the first tile's barrier
+  const P pol = gpol;
   const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   for (int64_t tile = blockIdx.x; tile < t.ntile; tile += gridDim.x) {
     const View v = tile_view(t, tile, D);
@@ -648,7 +625,7 @@ k_slab_mfma_adj(const Cf* __restrict__ psi, const Cf* __restrict__ lo_, Cf* __re
   extern __shared__ __align__(16) unsigned char smem[];
   Cf* shx = (Cf*)smem;
   Cf* shl = shx + t.cap;
-  const P pol = gpol.staged((unsigned char*)(shl + t.cap), true);  // synced by the first tile's barrier
+  const P pol = gpol;
   const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
   double g0 = 0.0, g1 = 0.0;
   for (int64_t tile = blockIdx.x; tile < t.ntile; tile += gridDim.x) {
@@ -706,9 +683,12 @@ bool plan_tile(Tile& t, int64_t N, int D, int64_t S_out, int64_t Sin, int64_t li
   return true;
 }
 
-// matrix-core tiles: 4096 amplitudes per array (leaves LDS for the staged gate
-// table), whole fiber blocks of 16 when a tile holds >= 16 slabs
-constexpr int64_t kMfmaTile = 4096;
+// matrix-core tiles: 8192 amplitudes per array (a whole slab of a DM mode pair
+// at cutoff 20, so each block operator serves twice the fibers of a half slab),
+// whole fiber blocks of 16 when a tile holds >= 16 slabs.  Operator entries are
+// read from the gate table in global memory (L1/L2-resident); staging it in LDS
+// cost more occupancy than it saved.
+constexpr int64_t kMfmaTile = 8192;
 bool plan_tile_mfma(Tile& t, int64_t N, int D, int64_t S_out, int64_t Sin, int64_t limit) {
   if (!plan_tile(t, N, D, S_out, Sin, limit)) return false;
   if (t.G >= 16) {
@@ -825,7 +805,7 @@ static int loss_slab_adj_launch(const void* psi, const void* lo, void* li, const
 // ---- complex64: matrix-core slab kernels
 template <int D, class P>
 static int slab_mfma_launch(const void* in, void* out, const Tile& t, const P& pol, hipStream_t s) {
-  const size_t lds = (size_t)t.cap * sizeof(Cf) + P::table_bytes(false);
+  const size_t lds = (size_t)t.cap * sizeof(Cf);
   RET_IF(allow_lds(k_slab_mfma<D, P>, lds));
   hipLaunchKernelGGL((k_slab_mfma<D, P>), dim3(slab_grid(t)), dim3(256), lds, s, (const Cf*)in, (Cf*)out, t, pol);
   return (int)hipGetLastError();
@@ -834,7 +814,7 @@ static int slab_mfma_launch(const void* in, void* out, const Tile& t, const P& p
 template <int D, class P>
 static int slab_mfma_adj_launch(const void* psi, const void* lo, void* li, const Tile& t, const P& pol,
                                 double* partial, int nwg, int zp1, hipStream_t s) {
-  const size_t lds = 2 * (size_t)t.cap * sizeof(Cf) + P::table_bytes(true);
+  const size_t lds = 2 * (size_t)t.cap * sizeof(Cf);
   RET_IF(allow_lds(k_slab_mfma_adj<D, P>, lds));
   hipLaunchKernelGGL((k_slab_mfma_adj<D, P>), dim3(nwg), dim3(512), lds, s, (const Cf*)psi, (const Cf*)lo, (Cf*)li,
                      t, pol, partial, nwg, zp1);
