@@ -152,6 +152,19 @@ inline int make_modelist(int n, const int* modes, int k, ModeList* ml) {
 // ---- internal launchers (implemented in the .hip files) ------------------
 namespace sfamd {
 
+// A run of consecutive rotation / Kerr gates applied in one pass (diag_run.hip).
+constexpr int kDiagRunMax = 8;
+struct DiagRun {
+  int32_t n, D, mixed, ngates;
+  int32_t mode[kDiagRunMax], kind[kDiagRunMax], op[kDiagRunMax];
+  int64_t toff[kDiagRunMax];    // table offset of each gate's D phases (complex elements)
+  int64_t rowdiv[kDiagRunMax];  // R^(n-2-mode): the gate's digit pair in a row index
+  int64_t N;                    // state elements
+};
+int diag_run_fwd(const void* in, void* out, const void* table, const DiagRun& r, int dtype, hipStream_t s);
+int diag_run_adj(const void* psi, const void* lam_out, void* lam_in, const void* table, const DiagRun& r,
+                 double* partial, int nwg, int dtype, hipStream_t s);
+
 int64_t table_size(int kind, int D);
 
 // one operation of a table build: kind and table offset; the operation's

@@ -277,8 +277,10 @@ struct Step {
   int fused;
   int op;  // single operation index (fused == 0)
   int K;
+  int diag;  // a run of rotation / Kerr gates in one pass (fused == 0)
   std::vector<int> ops;  // operation indices in execution order
   StageArgs args;
+  DiagRun run;
 };
 
 struct sfamd_prog {
@@ -387,7 +389,9 @@ void plan_run(sfamd_prog* p, const std::vector<int>& run, int Kc) {
     st.fused = 1;
     st.op = best.front();
     st.K = Kc;
+    st.diag = 0;
     st.ops = best;
+    memset(&st.run, 0, sizeof(st.run));
     StageArgs& a = st.args;
     memset(&a, 0, sizeof(a));
     a.ngates = (int)best.size();
@@ -449,21 +453,58 @@ void plan_steps(sfamd_prog* p) {
     if (!run.empty()) plan_run(p, run, Kc);
     run.clear();
   };
+  auto is_diag = [&](int i) {
+    const int k = p->ops[i].kind;
+    return k == SFAMD_OP_ROTATION || k == SFAMD_OP_KERR;
+  };
   for (int i = 0; i < p->nops; ++i) {
     if (gmax >= 1 && fusable(p->ops[i])) {
       run.push_back(i);
       continue;
     }
     flush();
+    // consecutive rotation / Kerr gates on the same representation share one pass
+    if (is_diag(i) && !p->steps.empty()) {
+      Step& last = p->steps.back();
+      if (!last.fused && is_diag(last.ops.back()) && p->ops[last.ops.back()].mixed_in == p->ops[i].mixed_in &&
+          (int)last.ops.size() < kDiagRunMax) {
+        last.ops.push_back(i);
+        last.diag = 1;
+        continue;
+      }
+    }
     Step st;
     st.fused = 0;
     st.op = i;
     st.K = 0;
+    st.diag = 0;
     st.ops.push_back(i);
     memset(&st.args, 0, sizeof(st.args));
+    memset(&st.run, 0, sizeof(st.run));
     p->steps.push_back(st);
   }
   flush();
+  for (Step& st : p->steps) {
+    if (!st.diag) continue;
+    DiagRun& r = st.run;
+    const OpPlan& first = p->ops[st.ops.front()];
+    r.n = n;
+    r.D = D;
+    r.mixed = first.mixed_in;
+    r.ngates = (int)st.ops.size();
+    r.N = ipow(D, r.mixed ? 2 * n : n);
+    const int64_t R = r.mixed ? (int64_t)D * D : D;
+    for (int g = 0; g < r.ngates; ++g) {
+      const OpPlan& op = p->ops[st.ops[g]];
+      r.mode[g] = op.m0;
+      r.kind[g] = op.kind;
+      r.op[g] = st.ops[g];
+      r.toff[g] = op.off;
+      int64_t dv = 1;
+      for (int t = 0; t < n - 2 - op.m0; ++t) dv *= R;
+      r.rowdiv[g] = dv;
+    }
+  }
   // state sizes at step inputs
   p->state_elems.clear();
   for (const Step& st : p->steps) {
@@ -708,6 +749,8 @@ extern "C" int sfamd_prog_forward(const sfamd_prog_t* p, int begin, int end, con
     if (st.fused) {
       int grid = (int)(st.args.ntiles < p->nwg ? st.args.ntiles : p->nwg);
       RET_IF(stage_fwd(p->D, st.K, cur, dst, table, st.args, grid, p->dtype, s));
+    } else if (st.diag) {
+      RET_IF(diag_run_fwd(cur, dst, table, st.run, p->dtype, s));
     } else {
       RET_IF(apply_op(p->ops[st.op], p->n, p->D, table, cur, dst, Ct, p->dtype, s));
     }
@@ -738,6 +781,8 @@ extern "C" int sfamd_prog_adjoint(const sfamd_prog_t* p, int begin, int end, con
     const Step& st = p->steps[i];
     if (st.fused) {
       RET_IF(stage_adj(p->D, st.K, saved[i - begin], lam, dst, table, st.args, nwg, partial, nwg, p->dtype, s));
+    } else if (st.diag) {
+      RET_IF(diag_run_adj(saved[i - begin], lam, dst, table, st.run, partial, nwg, p->dtype, s));
     } else {
       RET_IF(adjoint_op(p->ops[st.op], p->n, p->D, table, saved[i - begin], lam, dst, T1, T2,
                         partial + (int64_t)st.op * 4 * nwg, nwg, p->dtype, s));

@@ -1100,3 +1100,35 @@ def test_interferometer_matches_exact_passive_action(dtype, pure):
         got = to_np(res.state.ket() if pure else res.state.dm())
         want = ref if pure else fo.mix(ref, n)
         assert np.abs(got - want).max() < TOL[dtype] * 10, order
+
+
+@pytest.mark.parametrize("dtype", [C128, C64])
+@pytest.mark.parametrize("pure,n,D", [(True, 3, 5), (True, 2, 8), (False, 2, 4), (False, 3, 5)])
+def test_diagonal_runs_match_oracle_and_fd(dtype, pure, n, D):
+    """Consecutive R / K gates (repeated modes, more than one run's capacity)
+    run as one-pass diagonal runs: forward against the oracle, every gradient
+    against central differences; row lengths with and without whole groups of
+    four (vector and scalar paths)."""
+    rng = np.random.default_rng(500 + 10 * n + D)
+    ops = [("D", (m,), [float(rng.uniform(0.1, 0.4)), float(rng.uniform(0, 6))]) for m in range(n)]
+    for j in range(11):
+        ops.append(("R" if j % 2 == 0 else "K", (j % n,), [float(rng.uniform(-1, 1))]))
+    ops.append(("BS", (0, n - 1), [0.5, 0.3]))
+    ops += [("K", (m,), [float(rng.uniform(-0.5, 0.5))]) for m in range(n)]
+    cc = runtime.CompiledCircuit([(o[0], o[1]) for o in ops], n, D, not pure, dtype, DEV, fuse=0)
+    ref, rpure = fo.run_ops(ops, n, D, pure=pure)
+    rows = np.zeros((len(ops), 2))
+    for i, (_, _, p) in enumerate(ops):
+        rows[i, : len(p)] = p
+    params = torch.tensor(rows, dtype=torch.float64, device=DEV, requires_grad=True)
+    st0 = torch.as_tensor(fo.vacuum(n, D, pure=pure), dtype=dtype, device=DEV)
+    out = runtime.CircuitFunction.apply(params, st0, cc, None)
+    assert np.abs(to_np(out) - ref).max() < TOL[dtype] * 10
+    idx = np.indices([D] * n).sum(axis=0)
+    weights = (idx - 1.0) ** 2
+    (runtime.all_fock_probs(out, n, D, not rpure).to(torch.float64) * torch.as_tensor(weights, device=DEV)).sum().backward()
+    g = params.grad.cpu().numpy()
+    gflat = np.array([g[i, j] for i, (_, _, p) in enumerate(ops) for j in range(len(p))])
+    _, g_fd = _fd_grad(ops, n, D, pure, None, weights)
+    rel = np.linalg.norm(gflat - g_fd) / np.linalg.norm(g_fd)
+    assert rel < (1e-8 if dtype == C128 else 1e-5), rel
