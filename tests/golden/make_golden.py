@@ -131,8 +131,8 @@ def s2_mp(r, phi, D):
 
 
 CASES = {
-    "D": [(0.3, 0.7, 6), (-0.45, 2.1, 12), (0.9, -1.3, 20), (0.05, 0.0, 15)],
-    "S": [(0.3, 0.7, 6), (-0.4, 2.1, 12), (0.8, -1.3, 20), (0.1, 0.0, 15)],
+    "D": [(0.3, 0.7, 6), (-0.45, 2.1, 12), (0.9, -1.3, 20), (0.05, 0.0, 15), (0.9, 0.7, 64)],
+    "S": [(0.3, 0.7, 6), (-0.4, 2.1, 12), (0.8, -1.3, 20), (0.1, 0.0, 15), (0.6, -0.4, 64)],
     "BS": [(0.6, 0.4, 5), (1.1, -2.3, 8), (-0.35, 1.7, 12), (0.6, 0.4, 20)],
     "S2": [(0.3, 0.7, 6), (-0.4, 2.1, 8), (0.9, -1.3, 12), (0.6, 0.4, 20)],
 }

@@ -1132,3 +1132,68 @@ def test_diagonal_runs_match_oracle_and_fd(dtype, pure, n, D):
     _, g_fd = _fd_grad(ops, n, D, pure, None, weights)
     rel = np.linalg.norm(gflat - g_fd) / np.linalg.norm(g_fd)
     assert rel < (1e-8 if dtype == C128 else 1e-5), rel
+
+
+# ------------------------------------------------------------------ maximum cutoffs
+@pytest.mark.parametrize("dtype", [C128, C64])
+def test_maximum_cutoff_tables_and_ops(dtype):
+    """The largest cutoff of each gate family (64 single-mode, 40 two-mode, 32
+    cubic phase): tables against the oracle and gates on 1- and 2-mode states,
+    pure and mixed (the element-wise kernels past the matrix-core range)."""
+    tol_t = 1e-10 if dtype == C128 else 2e-6
+    # at cutoff 64 the FP64 recurrences (library and oracle alike) carry up to ~2e-9
+    # (squeezing): compare with the 50-digit closed forms of tests/golden
+    g = np.load(os.path.join(GOLDEN, "gates_mpmath.npz"))
+    for key in ("D_4", "S_4"):
+        p0, p1, Dg = g[key + "_params"]
+        assert int(Dg) == 64
+        t = to_np(runtime.build_gate(key[0], p0, p1, 64, dtype, DEV))
+        assert np.abs(t[: 64 * 64].reshape(64, 64) - g[key]).max() < (5e-9 if dtype == C128 else 2e-6), key
+    t = to_np(runtime.build_gate("V", 0.4, 0.0, 32, dtype, DEV))
+    assert np.abs(t[: 32 * 32].reshape(32, 32) - fo.cubic_phase(0.4, 32)).max() < tol_t
+    # the two-mode recurrences (library and oracle alike) lose accuracy with the
+    # photon number: against 50-digit closed forms at cutoff 40 the oracle's
+    # beamsplitter is off by 1e-14 at N1 + N2 = 20, 1e-10 at 50 and 7e-7 at 78,
+    # its S2 by 7e-12 at out1 + in2 = 40 and 8e-7 at 78.  Entries below 40
+    # photons are held to the FP64 floor, the rest to the conditioning.
+    bs_tol = 5e-6 if dtype == C128 else 1e-5
+    low_tol = 1e-10 if dtype == C128 else 2e-6
+    o1, i1, o2, i2 = np.indices((40,) * 4)
+    for kind, fn, unpack, low in [
+        ("BS", lambda a, b, D: fo.beamsplitter(a, b, D), _bs_dense_from_blocks, (o1 + o2) < 40),
+        ("S2", fo.two_mode_squeezing, _s2_dense_from_blocks, (o1 + i2) < 40),
+        ("CK", lambda a, b, D: fo.cross_kerr(a, D), _bs_dense_from_blocks, (o1 + o2) < 80),
+    ]:
+        t = to_np(runtime.build_gate(kind, 0.6, 0.4, 40, dtype, DEV))
+        d = np.abs(unpack(t, 40)[0] - fn(0.6, 0.4, 40))
+        assert d.max() < bs_tol, kind
+        assert d[low].max() < low_tol, kind
+    # MZ at cutoff 40: complete photon-number blocks unitary, one-photon block the mode unitary
+    t = to_np(runtime.build_gate("MZ", 0.3, 0.7, 40, dtype, DEV))
+    M, _ = _bs_dense_from_blocks(t, 40)
+    for s in (1, 7, 39):
+        ks = np.arange(s + 1)
+        blk = M[ks[:, None], ks[None, :], (s - ks)[:, None], (s - ks)[None, :]]
+        assert np.abs(blk @ blk.conj().T - np.eye(s + 1)).max() < (1e-11 if dtype == C128 else 1e-5), s
+    assert np.abs(M[:30, :30, :30, :30] - fo.mzgate(0.3, 0.7, 30)).max() < bs_tol
+    tol64 = 1e-8 if dtype == C128 else TOL[dtype] * 10  # cutoff-64 table conditioning (above)
+    for pure in (True, False):
+        st1 = rand_state(1, 64, pure, 31)
+        for name, p in [("R", (0.7,)), ("K", (0.2,)), ("D", (0.5, 1.1)), ("S", (0.3, 0.4))] + ([] if pure else [("LOSS", (0.7,))]):
+            ops = [(name, (0,), p)]
+            got = run_backend(ops, 1, 64, dtype, pure=pure, init=st1).state()
+            ref, rpure = fo.run_ops(ops, 1, 64, state=st1)
+            assert np.abs(to_np(got.ket() if rpure else got.dm()) - ref).max() < tol64, (name, pure)
+        ops = [("V", (0,), (0.3,))]
+        st32 = rand_state(1, 32, pure, 32)
+        got = run_backend(ops, 1, 32, dtype, pure=pure, init=st32).state()
+        ref, rpure = fo.run_ops(ops, 1, 32, state=st32)
+        assert np.abs(to_np(got.ket() if rpure else got.dm()) - ref).max() < TOL[dtype] * 10, ("V", pure)
+        st2 = rand_state(2, 40, pure, 33)
+        for name, p in [("BS", (0.6, 0.4)), ("S2", (0.2, 0.5)), ("CK", (0.4,))]:
+            for modes in [(0, 1), (1, 0)]:
+                ops = [(name, modes, p)]
+                got = run_backend(ops, 2, 40, dtype, pure=pure, init=st2).state()
+                ref, rpure = fo.run_ops(ops, 2, 40, state=st2)
+                tol = (1e-7 if dtype == C128 else 2e-5) if name in ("BS", "S2") else TOL[dtype] * 10
+                assert np.abs(to_np(got.ket() if rpure else got.dm()) - ref).max() < tol, (name, modes, pure)
