@@ -54,10 +54,11 @@ def parse():
     ap.add_argument(
         "--workload",
         default="c3",
-        choices=["c2", "c3", "c4", "c5", "hbm"],
+        choices=["c2", "c3", "c4", "c5", "hbm", "next"],
         help="c3: 6-mode ket gradient (default line); c4: 256 uneven circuits in one graph launch; "
         "c5: 3-mode cutoff-20 density-matrix gradient with loss and layer checkpoints; "
-        "hbm: per-kernel HBM rates on a 2 GB ket and the 512 MB DM",
+        "hbm: per-kernel HBM rates on a 2 GB ket and the 512 MB DM; "
+        "next: S2/MZ/CK/V gates, diagonal runs and measurement kernels on the same states",
     )
     return ap.parse_args()
 
@@ -78,10 +79,12 @@ def _event_us(fn, reps, stream=None):
     return e0.elapsed_time(e1) * 1e3 / reps
 
 
-def kernel_rates(n, D, mixed, dtype, reps=10):
+def kernel_rates(n, D, mixed, dtype, reps=10, kinds=None):
     """Average launch time and algorithmic GB/s of each operation kind on one
     state: forward 2 S per operation (two passes on a DM for D, S, BS), adjoint
-    4 S (forward state, cotangent in, cotangent out, + recompute on a DM)."""
+    4 S (forward state, cotangent in, cotangent out, + recompute on a DM).
+    `kinds`: [(name, modes)] or [(name, modes, (p0, p1))]; default the gates of
+    the main sweep."""
     import torch
 
     from strawberryfields_amd import runtime
@@ -94,17 +97,20 @@ def kernel_rates(n, D, mixed, dtype, reps=10):
     out = torch.empty_like(st)
     lam = torch.randn_like(st)
     gin = torch.empty_like(st)
-    kinds = [("D", (0,)), ("D", (n - 1,)), ("S", (n // 2,)), ("R", (0,)), ("K", (n - 1,)), ("BS", (0, 1)), ("BS", (n - 2, n - 1))]
-    if mixed:
-        kinds += [("LOSS", (0,)), ("LOSS", (n - 1,))]
+    if kinds is None:
+        kinds = [("D", (0,)), ("D", (n - 1,)), ("S", (n // 2,)), ("R", (0,)), ("K", (n - 1,)), ("BS", (0, 1)), ("BS", (n - 2, n - 1))]
+        if mixed:
+            kinds += [("LOSS", (0,)), ("LOSS", (n - 1,))]
     res = {}
     # plain device copy of the same state: the bandwidth reference (and the
     # kernel whose HBM counters calibrate FETCH_SIZE / WRITE_SIZE)
     tc = _event_us(lambda: out.copy_(st), reps)
     res["copy"] = {"fwd_us": round(tc, 1), "fwd_GBs": round(2 * S / tc / 1e3, 1)}
-    for kind, modes in kinds:
+    for entry in kinds:
+        kind, modes = entry[0], entry[1]
+        par = entry[2] if len(entry) > 2 else ((0.3, 0.2) if kind != "LOSS" else (0.8, 0.0))
         cc = runtime.CompiledCircuit([(kind, modes)], n, D, mixed, dtype, dev, fuse=0)
-        cc.build_table(torch.tensor([[0.3, 0.2] if kind != "LOSS" else [0.8, 0.0]], dtype=torch.float64, device=dev))
+        cc.build_table(torch.tensor([list(par)], dtype=torch.float64, device=dev))
         grads = torch.zeros(2, dtype=torch.float64, device=dev)
         tf = _event_us(lambda: cc.forward(0, 1, st, out=out), reps)
         ta = _event_us(lambda: cc.adjoint(0, 1, [st], lam, gin, grads), reps)
@@ -338,6 +344,73 @@ def run_hbm(args):
     )
 
 
+def run_next(args):
+    """The operations past the core gate set, on the same large states as the HBM
+    sweep (complex64): S2, MZ, CK and V gates (forward and adjoint), a run of
+    three R / K gates in one pass, and the measurement / preparation kernels
+    (marginal distribution, |0><x| projection, homodyne density, multi-mode
+    embedding)."""
+    import torch
+
+    from strawberryfields_amd import runtime
+
+    dev = torch.device("cuda:0")
+    c64 = torch.complex64
+    ket_kinds = [("S2", (0, 1)), ("S2", (5, 6)), ("MZ", (0, 1)), ("MZ", (5, 6)), ("CK", (0, 1)), ("V", (0,), (0.2, 0.0))]
+    dm_kinds = [("S2", (0, 1)), ("S2", (1, 2)), ("MZ", (1, 2)), ("CK", (0, 1)), ("V", (0,), (0.2, 0.0))]
+    S1, r1 = kernel_rates(7, 16, False, c64, kinds=ket_kinds)
+    S2, r2 = kernel_rates(3, 20, True, c64, kinds=dm_kinds)
+    meas = {}
+    for label, n, D, mixed in [("ket_7modes_cutoff16", 7, 16, False), ("dm_3modes_cutoff20", 3, 20, True)]:
+        shape = [D] * (2 * n if mixed else n)
+        st = torch.randn(shape, dtype=c64, device=dev)
+        S = st.numel() * 8
+        m = {}
+        # a run of three diagonal gates on three modes, one pass
+        ops = [("R", (0,)), ("K", (1,)), ("R", (n - 1,))]
+        cc = runtime.CompiledCircuit(ops, n, D, mixed, c64, dev, fuse=0)
+        cc.build_table(torch.tensor([[0.3, 0.0]] * 3, dtype=torch.float64, device=dev))
+        out = torch.empty_like(st)
+        lam = torch.randn_like(st)
+        gin = torch.empty_like(st)
+        grads = torch.zeros(6, dtype=torch.float64, device=dev)
+        tf = _event_us(lambda: cc.forward(0, cc.nsteps, st, out=out), 10)
+        ta = _event_us(lambda: cc.adjoint(0, cc.nsteps, [st], lam, gin, grads), 10)
+        m["R,K,R run (one pass)"] = {
+            "fwd_us": round(tf, 1),
+            "fwd_pct_peak": round(100 * 2 * S / tf / 1e3 / HBM_PEAK_GBS, 1),
+            "adj_us": round(ta, 1),
+            "adj_pct_peak": round(100 * 3 * S / ta / 1e3 / HBM_PEAK_GBS, 1),
+        }
+        for k in (1, 2):
+            t = _event_us(lambda: runtime.marginal_probs(st, n, D, mixed, list(range(k))), 10)
+            read = (D**n) * 8 if mixed else S  # a DM's marginal reads its diagonal only
+            m["marginal_probs %d mode(s)" % k] = {"us": round(t, 1), "GBs_read": round(read / t / 1e3, 1)}
+        t = _event_us(lambda: runtime.project_reset(st, n, D, mixed, [0], [1], 1.0), 10)
+        m["project_reset 1 mode"] = {"us": round(t, 1), "pct_peak": round(100 * 2 * S / t / 1e3 / HBM_PEAK_GBS, 1)}
+        if mixed:
+            rho = runtime.reduced_dm(st, n, D, True, [1])
+            t = _event_us(lambda: runtime.homodyne_pdf(rho, D, 0.3, 10.0, 100000, 2.0), 10)
+            m["homodyne_pdf 100000 bins"] = {"us": round(t, 1), "GFLOPs": round(100000 * D * D * 2 / t / 1e3, 1)}
+            rest = runtime.reduced_dm(st, n, D, True, [0])
+            sig = torch.randn([D] * 4, dtype=c64, device=dev)
+            t = _event_us(lambda: runtime.embed_modes(rest, sig, n, D, [2, 1], True), 10)
+            m["embed 2-mode DM"] = {"us": round(t, 1), "pct_peak": round(100 * S / t / 1e3 / HBM_PEAK_GBS, 1)}
+        meas[label] = m
+        del st, out, lam, gin
+        torch.cuda.empty_cache()
+    print(
+        json.dumps(
+            {
+                "metric": "S2/MZ/CK/V gates, diagonal runs and measurement kernels on large states (peak %d GB/s)" % HBM_PEAK_GBS,
+                "ket_7modes_cutoff16": {"state_MB": S1 / 2**20, "gates": r1},
+                "dm_3modes_cutoff20": {"state_MB": S2 / 2**20, "gates": r2},
+                "measurement": meas,
+            }
+        )
+    )
+
+
 def stage_adj_traffic():
     """HBM bytes per k_stage_adj launch measured by the FETCH_SIZE / WRITE_SIZE
     counter runs recorded in profiles/ (calibrated against a device copy)."""
@@ -416,6 +489,8 @@ def main():
         return run_c2(args)
     if args.workload == "hbm":
         return run_hbm(args)
+    if args.workload == "next":
+        return run_next(args)
     c3 = circuits.config3()
     n, D, ops = c3["n"], c3["D"], c3["ops"]
     dtype = torch.complex64

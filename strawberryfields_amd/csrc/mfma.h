@@ -16,12 +16,14 @@ int axis_dense_adj_mfma(const void* psi, const void* lam_out, void* lam_in, int6
                         hipStream_t s);
 
 // beamsplitter on axes of strides sp (first mode) and sq, blocks in the compact
-// layout of the gate table; use when the inner gate axis run is >= 16
+// layout of the gate table; use when the inner gate axis run is >= 16.
+// diff = 1: difference-conserving blocks (two-mode squeezing, block s holds
+// n1 - n2 = s - (D - 1)) instead of photon-number blocks.
 int pair_bs_mfma(const void* in, void* out, int64_t N, int D, int64_t sp, int64_t sq, const void* blocks, int conj,
-                 hipStream_t s);
+                 hipStream_t s, int diff = 0);
 int pair_bs_adj_mfma(const void* psi, const void* lam_out, void* lam_in, int64_t N, int D, int64_t sp, int64_t sq,
                      const void* blocks, const void* dblocks, int conj, int wa, int wb, double* partial, int nwg,
-                     int zp1, hipStream_t s);
+                     int zp1, hipStream_t s, int diff = 0);
 
 // rho' = U rho U^H on one mode of a DM (bra stride D * sk, ket stride sk) in
 // one pass; supported for sk == 1 or sk >= 16

@@ -186,24 +186,34 @@ __device__ __forceinline__ void mfma_write_partials(double g0, double g1, double
   }
 }
 
+// Fibers of block s.  Photon-number blocks (BS, MZ, CK: n1 + n2 = s) hold the
+// amplitudes (k, s - k); difference blocks (S2, diff = 1: n1 - n2 = d,
+// d = s - (D - 1)) hold (k, k - d), for k = lo .. lo + c - 1.  Offset of the
+// first amplitude from the fiber base, and the step between amplitudes:
+__device__ __forceinline__ int64_t block_first(int64_t sp, int64_t sq, int lo, int s, int D, int diff) {
+  return lo * sp + (int64_t)(diff ? lo - (s - (D - 1)) : s - lo) * sq;
+}
+__device__ __forceinline__ int64_t block_step(int64_t sp, int64_t sq, int diff) { return diff ? sp + sq : sp - sq; }
+
 // one task of a beamsplitter pass: block s (c amplitudes from N = lo) on the
 // fibers f0 .. f0 + kBsChunk - 1
 template <int MB, int D>
 __device__ __forceinline__ void bs_task(const C* __restrict__ in, C* __restrict__ out, int64_t NF, const PairGeo& geo,
                                         int64_t sp, int64_t sq, const C* __restrict__ bb, int c, int lo, int s,
-                                        int conj, int64_t f0) {
+                                        int conj, int diff, int64_t f0) {
   constexpr int Q = Blk<MB>::Q;
   float A[MB][Blk<MB>::NS];
   load_op<MB>([&](int i, int k) { return op_entry(bb, c, conj ? 1 : 0, i, k); }, A);
   const int lane = threadIdx.x & 63;
   const int cl = lane & 15, g = lane >> 4;
-  const int64_t step = sp - sq;
+  const int64_t step = block_step(sp, sq, diff);
+  const int64_t first = block_first(sp, sq, lo, s, D, diff);
   const int64_t left = NF - f0;
   const int nf = left < kBsChunk ? (int)left : kBsChunk;
   for (int b16 = threadIdx.x >> 6; b16 * 16 < nf; b16 += blockDim.x >> 6) {
     const bool valid = b16 * 16 + cl < nf;
     const int64_t f = f0 + (valid ? b16 * 16 + cl : 0);
-    const int64_t base = pair_base(f, geo, D) + lo * sp + (s - lo) * sq + (int64_t)(g * Q) * step;
+    const int64_t base = pair_base(f, geo, D) + first + (int64_t)(g * Q) * step;
     C x[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) x[q] = (valid && g * Q + q < c) ? in[base + q * step] : mk<float>(0.f, 0.f);
@@ -220,7 +230,7 @@ __device__ __forceinline__ void bs_task(const C* __restrict__ in, C* __restrict_
 template <int D>
 __global__ void __launch_bounds__(256)
 k_bs_mfma(const C* __restrict__ in, C* __restrict__ out, int64_t NF, PairGeo geo, int64_t sp, int64_t sq,
-          const C* __restrict__ blk, int conj) {
+          const C* __restrict__ blk, int conj, int diff) {
   constexpr int NSB = 2 * D - 1;
   const int64_t ntask = (NF + kBsChunk - 1) / kBsChunk * NSB;
   for (int64_t task = blockIdx.x; task < ntask; task += gridDim.x) {
@@ -231,12 +241,12 @@ k_bs_mfma(const C* __restrict__ in, C* __restrict__ out, int64_t NF, PairGeo geo
     const C* bb = blk + bs_block_off(s, D);
     const int mb = (2 * c + 15) / 16;
     if (mb == 1)
-      bs_task<1, D>(in, out, NF, geo, sp, sq, bb, c, lo, s, conj, f0);
+      bs_task<1, D>(in, out, NF, geo, sp, sq, bb, c, lo, s, conj, diff, f0);
     else if constexpr (D > 8) {
       if (mb == 2)
-        bs_task<2, D>(in, out, NF, geo, sp, sq, bb, c, lo, s, conj, f0);
+        bs_task<2, D>(in, out, NF, geo, sp, sq, bb, c, lo, s, conj, diff, f0);
       else if constexpr (D > 16)
-        bs_task<3, D>(in, out, NF, geo, sp, sq, bb, c, lo, s, conj, f0);
+        bs_task<3, D>(in, out, NF, geo, sp, sq, bb, c, lo, s, conj, diff, f0);
     }
   }
 }
@@ -248,7 +258,7 @@ template <int MB, int D>
 __device__ __forceinline__ void bs_adj_task(const C* __restrict__ psi, const C* __restrict__ lo_, C* __restrict__ li,
                                             int64_t NF, const PairGeo& geo, int64_t sp, int64_t sq,
                                             const C* __restrict__ bb, const C* __restrict__ db, int c, int lo, int s,
-                                            int conj, int wa, int wb, int64_t f0, float* opl, double& g0,
+                                            int conj, int wa, int wb, int diff, int64_t f0, float* opl, double& g0,
                                             double& g1) {
   constexpr int Q = Blk<MB>::Q, OPN = MB * Blk<MB>::NS * 64;
   __syncthreads();  // the previous task's operators are no longer read
@@ -265,14 +275,15 @@ __device__ __forceinline__ void bs_adj_task(const C* __restrict__ psi, const C* 
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int cl = lane & 15, g = lane >> 4;
-  const int64_t step = sp - sq;
+  const int64_t step = block_step(sp, sq, diff);
+  const int64_t first = block_first(sp, sq, lo, s, D, diff);
   const int64_t left = NF - f0;
   const int nf = left < kBsChunk ? (int)left : kBsChunk;
   for (int b16 = threadIdx.x >> 6; b16 * 16 < nf; b16 += blockDim.x >> 6) {
     __asm__ volatile("" ::: "memory");  // operators stay in LDS (see k_axis_mfma_adj)
     const bool valid = b16 * 16 + cl < nf;
     const int64_t f = f0 + (valid ? b16 * 16 + cl : 0);
-    const int64_t base = pair_base(f, geo, D) + lo * sp + (s - lo) * sq + (int64_t)(g * Q) * step;
+    const int64_t base = pair_base(f, geo, D) + first + (int64_t)(g * Q) * step;
     C x[Q], l[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
@@ -303,7 +314,7 @@ template <int D>
 __global__ void __launch_bounds__(256)
 k_bs_mfma_adj(const C* __restrict__ psi, const C* __restrict__ lo_, C* __restrict__ li, int64_t NF, PairGeo geo,
               int64_t sp, int64_t sq, const C* __restrict__ blk, const C* __restrict__ dblk, int conj, int wa,
-              int wb, double* __restrict__ partial, int nwg, int zp1) {
+              int wb, int diff, double* __restrict__ partial, int nwg, int zp1) {
   constexpr int NSB = 2 * D - 1, MBX = Plan<D>::Mb;
   __shared__ float opl[3 * MBX * Blk<MBX>::NS * 64];
   const int64_t ntask = (NF + kBsChunk - 1) / kBsChunk * NSB;
@@ -316,14 +327,14 @@ k_bs_mfma_adj(const C* __restrict__ psi, const C* __restrict__ lo_, C* __restric
     const int64_t off = bs_block_off(s, D);
     const int mb = (2 * c + 15) / 16;
     if (mb == 1)
-      bs_adj_task<1, D>(psi, lo_, li, NF, geo, sp, sq, blk + off, dblk + off, c, lo, s, conj, wa, wb, f0, opl, g0,
+      bs_adj_task<1, D>(psi, lo_, li, NF, geo, sp, sq, blk + off, dblk + off, c, lo, s, conj, wa, wb, diff, f0, opl, g0,
                           g1);
     else if constexpr (D > 8) {
       if (mb == 2)
-        bs_adj_task<2, D>(psi, lo_, li, NF, geo, sp, sq, blk + off, dblk + off, c, lo, s, conj, wa, wb, f0, opl, g0,
+        bs_adj_task<2, D>(psi, lo_, li, NF, geo, sp, sq, blk + off, dblk + off, c, lo, s, conj, wa, wb, diff, f0, opl, g0,
                           g1);
       else if constexpr (D > 16)
-        bs_adj_task<3, D>(psi, lo_, li, NF, geo, sp, sq, blk + off, dblk + off, c, lo, s, conj, wa, wb, f0, opl, g0,
+        bs_adj_task<3, D>(psi, lo_, li, NF, geo, sp, sq, blk + off, dblk + off, c, lo, s, conj, wa, wb, diff, f0, opl, g0,
                           g1);
     }
   }
@@ -484,7 +495,7 @@ static PairGeo pair_geo(int64_t sp, int64_t sq, int D) {
 }
 
 int pair_bs_mfma(const void* in, void* out, int64_t N, int D, int64_t sp, int64_t sq, const void* blocks, int conj,
-                 hipStream_t s) {
+                 hipStream_t s, int diff) {
   const int64_t NF = N / ((int64_t)D * D);
   const PairGeo g = pair_geo(sp, sq, D);
   int64_t ntask = (NF + kBsChunk - 1) / kBsChunk * (2 * D - 1);
@@ -493,7 +504,7 @@ int pair_bs_mfma(const void* in, void* out, int64_t N, int D, int64_t sp, int64_
 #define CASE(d)                                                                                                 \
   case d:                                                                                                       \
     hipLaunchKernelGGL((k_bs_mfma<d>), dim3(grid), dim3(kBlock), 0, s, (const C*)in, (C*)out, NF, g, sp, sq,   \
-                       (const C*)blocks, conj);                                                                 \
+                       (const C*)blocks, conj, diff);                                                           \
     break;
     SFAMD_MFMA_D(CASE)
 #undef CASE
@@ -505,15 +516,15 @@ int pair_bs_mfma(const void* in, void* out, int64_t N, int D, int64_t sp, int64_
 
 int pair_bs_adj_mfma(const void* psi, const void* lam_out, void* lam_in, int64_t N, int D, int64_t sp, int64_t sq,
                      const void* blocks, const void* dblocks, int conj, int wa, int wb, double* partial, int nwg,
-                     int zp1, hipStream_t s) {
+                     int zp1, hipStream_t s, int diff) {
   const int64_t NF = N / ((int64_t)D * D);
   const PairGeo g = pair_geo(sp, sq, D);
   switch (D) {
 #define CASE(d)                                                                                                     \
   case d:                                                                                                           \
     hipLaunchKernelGGL((k_bs_mfma_adj<d>), dim3(nwg), dim3(kBlock), 0, s, (const C*)psi, (const C*)lam_out,        \
-                       (C*)lam_in, NF, g, sp, sq, (const C*)blocks, (const C*)dblocks, conj, wa, wb, partial, nwg,  \
-                       zp1);                                                                                       \
+                       (C*)lam_in, NF, g, sp, sq, (const C*)blocks, (const C*)dblocks, conj, wa, wb, diff, partial, \
+                       nwg, zp1);                                                                                  \
     break;
     SFAMD_MFMA_D(CASE)
 #undef CASE

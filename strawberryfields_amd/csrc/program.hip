@@ -122,14 +122,29 @@ int bs_adj_pass(const void* psi, const void* lo, void* li, int64_t N, int D, int
 // beamsplitter family)
 int pair_pass(int kind, const void* in, void* out, int64_t N, int D, int64_t sp, int64_t sq, const void* g, int conj,
               int dtype, hipStream_t s) {
-  if (kind == SFAMD_OP_TWO_MODE_SQUEEZE) return pair_s2(in, out, N, D, sp, sq, g, conj, dtype, s);
+  if (kind == SFAMD_OP_TWO_MODE_SQUEEZE) {
+    // complex64: the beamsplitter's matrix-core and slab kernels on difference blocks
+    if (mfma_supported(D, dtype)) {
+      if (use_fib(D, sp < sq ? sp : sq)) return pair_bs_mfma(in, out, N, D, sp, sq, g, conj, s, 1);
+      if (use_slab(N, D, sp, sq, dtype, 0)) return pair_bs_slab(in, out, N, D, sp, sq, g, conj, dtype, s, 1);
+    }
+    return pair_s2(in, out, N, D, sp, sq, g, conj, dtype, s);
+  }
   return bs_pass(in, out, N, D, sp, sq, g, conj, dtype, s);
 }
 
 int pair_adj_pass(int kind, const void* psi, const void* lo, void* li, int64_t N, int D, int64_t sp, int64_t sq,
                   const void* g, const void* dG, int conj, double* pb, int nwg, int zp1, int dtype, hipStream_t s) {
-  if (kind == SFAMD_OP_TWO_MODE_SQUEEZE)
+  if (kind == SFAMD_OP_TWO_MODE_SQUEEZE) {
+    // d/dphi G[N][n] = i (N - n) G[N][n]: weights wa = -1, wb = 1
+    if (mfma_supported(D, dtype)) {
+      if (use_fib(D, sp < sq ? sp : sq))
+        return pair_bs_adj_mfma(psi, lo, li, N, D, sp, sq, g, dG, conj, -1, 1, pb, nwg, zp1, s, 1);
+      if (use_slab(N, D, sp, sq, dtype, 1))
+        return pair_bs_adj_slab(psi, lo, li, N, D, sp, sq, g, dG, conj, -1, 1, pb, nwg, zp1, dtype, s, 1);
+    }
     return pair_s2_adj(psi, lo, li, N, D, sp, sq, g, dG, conj, pb, nwg, zp1, dtype, s);
+  }
   int wa, wb;
   bs_weights(kind, &wa, &wb);
   return bs_adj_pass(psi, lo, li, N, D, sp, sq, g, dG, conj, wa, wb, pb, nwg, zp1, dtype, s);

@@ -159,7 +159,8 @@ struct BsFiber {
   int pos0, step;
 };
 
-__device__ __forceinline__ BsFiber bs_fiber(int phi, int per_s, const Tile& t, const View& v, int p_outer, int D) {
+__device__ __forceinline__ BsFiber bs_fiber(int phi, int per_s, const Tile& t, const View& v, int p_outer, int D,
+                                            int diff = 0) {
   BsFiber b;
   b.s = phi / per_s;
   const int r = phi - b.s * per_s;
@@ -171,8 +172,10 @@ __device__ __forceinline__ BsFiber bs_fiber(int phi, int per_s, const Tile& t, c
   const int lp = p_outer ? sA : t.Sin, lq = p_outer ? t.Sin : sA;
   b.lo = bs_lo(b.s, D);
   b.c = bs_hi(b.s, D) - b.lo + 1;
-  b.pos0 = gsl * D * sA + m * t.RW + cidx + b.lo * lp + (b.s - b.lo) * lq;
-  b.step = lp - lq;
+  // photon-number block: amplitudes (k, s - k); difference block (diff): (k, k - d)
+  const int q0 = diff ? b.lo - (b.s - (D - 1)) : b.s - b.lo;
+  b.pos0 = gsl * D * sA + m * t.RW + cidx + b.lo * lp + q0 * lq;
+  b.step = diff ? lp + lq : lp - lq;
   return b;
 }
 
@@ -457,10 +460,11 @@ struct BsPol {
   const Cf* dblk;
   int conj, p_outer;
   int wa, wb;  // second-parameter derivative weight w = wa n + wb N
+  int diff;    // difference-conserving blocks (two-mode squeezing)
   __device__ int size(int s) const { return bs_hi(s, D) - bs_lo(s, D) + 1; }
   __device__ int per_block(const Tile& t, const View& v) const { return v.gs * v.mt * t.Sin; }
   __device__ void fiber(int s, int phi, const Tile& t, const View& v, int& pos0, int& step) const {
-    const BsFiber f = bs_fiber(s * per_block(t, v) + phi, per_block(t, v), t, v, p_outer, D);
+    const BsFiber f = bs_fiber(s * per_block(t, v) + phi, per_block(t, v), t, v, p_outer, D, diff);
     pos0 = f.pos0;
     step = f.step;
   }
@@ -822,14 +826,14 @@ static int slab_mfma_adj_launch(const void* psi, const void* lo, void* li, const
 }
 
 static int bs_slab_mfma(const void* in, void* out, int64_t N, int D, int64_t sp, int64_t sq, const void* blocks,
-                        int conj, hipStream_t s) {
+                        int conj, int diff, hipStream_t s) {
   Tile t;
   const int p_outer = sp > sq;
   if (!plan_tile_mfma(t, N, D, p_outer ? sp : sq, p_outer ? sq : sp, kMfmaTile)) return SFAMD_E_ARG;
   switch (D) {
 #define CASE(d)                                                     \
   case d: {                                                         \
-    BsPol<d> pol{(const Cf*)blocks, nullptr, conj, p_outer, 0, 0};  \
+    BsPol<d> pol{(const Cf*)blocks, nullptr, conj, p_outer, 0, 0, diff}; \
     return slab_mfma_launch<d>(in, out, t, pol, s);                 \
   }
     SFAMD_SLAB_D(CASE)
@@ -840,15 +844,15 @@ static int bs_slab_mfma(const void* in, void* out, int64_t N, int D, int64_t sp,
 }
 
 static int bs_slab_mfma_adj(const void* psi, const void* lo, void* li, int64_t N, int D, int64_t sp, int64_t sq,
-                            const void* blocks, const void* dblocks, int conj, int wa, int wb, double* partial,
-                            int nwg, int zp1, hipStream_t s) {
+                            const void* blocks, const void* dblocks, int conj, int wa, int wb, int diff,
+                            double* partial, int nwg, int zp1, hipStream_t s) {
   Tile t;
   const int p_outer = sp > sq;
   if (!plan_tile_mfma(t, N, D, p_outer ? sp : sq, p_outer ? sq : sp, kMfmaTile)) return SFAMD_E_ARG;
   switch (D) {
 #define CASE(d)                                                                    \
   case d: {                                                                        \
-    BsPol<d> pol{(const Cf*)blocks, (const Cf*)dblocks, conj, p_outer, wa, wb};    \
+    BsPol<d> pol{(const Cf*)blocks, (const Cf*)dblocks, conj, p_outer, wa, wb, diff}; \
     return slab_mfma_adj_launch<d>(psi, lo, li, t, pol, partial, nwg, zp1, s);     \
   }
     SFAMD_SLAB_D(CASE)
@@ -893,8 +897,9 @@ static int loss_slab_mfma_adj(const void* psi, const void* lo, void* li, int64_t
 }
 
 int pair_bs_slab(const void* in, void* out, int64_t N, int D, int64_t sp, int64_t sq, const void* blocks, int conj,
-                 int dtype, hipStream_t s) {
-  if (dtype == SFAMD_C64) return bs_slab_mfma(in, out, N, D, sp, sq, blocks, conj, s);
+                 int dtype, hipStream_t s, int diff) {
+  if (dtype == SFAMD_C64) return bs_slab_mfma(in, out, N, D, sp, sq, blocks, conj, diff, s);
+  if (diff) return SFAMD_E_ARG;  // difference blocks: matrix-core variant only
   Tile t;
   const int p_outer = sp > sq;
   if (!plan_tile(t, N, D, p_outer ? sp : sq, p_outer ? sq : sp, slab_limit(dtype, 0))) return SFAMD_E_ARG;
@@ -912,9 +917,11 @@ int pair_bs_slab(const void* in, void* out, int64_t N, int D, int64_t sp, int64_
 
 int pair_bs_adj_slab(const void* psi, const void* lam_out, void* lam_in, int64_t N, int D, int64_t sp, int64_t sq,
                      const void* blocks, const void* dblocks, int conj, int wa, int wb, double* partial, int nwg,
-                     int zp1, int dtype, hipStream_t s) {
+                     int zp1, int dtype, hipStream_t s, int diff) {
   if (dtype == SFAMD_C64)
-    return bs_slab_mfma_adj(psi, lam_out, lam_in, N, D, sp, sq, blocks, dblocks, conj, wa, wb, partial, nwg, zp1, s);
+    return bs_slab_mfma_adj(psi, lam_out, lam_in, N, D, sp, sq, blocks, dblocks, conj, wa, wb, diff, partial, nwg, zp1,
+                            s);
+  if (diff) return SFAMD_E_ARG;
   Tile t;
   const int p_outer = sp > sq;
   if (!plan_tile(t, N, D, p_outer ? sp : sq, p_outer ? sq : sp, slab_limit(dtype, 1))) return SFAMD_E_ARG;
