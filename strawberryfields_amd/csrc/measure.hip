@@ -28,20 +28,24 @@ int64_t marginal_chunks(int64_t NR) {
 
 // p(bin) = sum over the traced modes of |psi|^2 (ket) or rho diagonal (DM);
 // grid.x = bin (kept-mode digits, first kept mode most significant), grid.y =
-// chunk of the traced index range.  FP64 accumulation.
+// chunk of the traced index range.  FP64 accumulation.  The strides of each
+// digit (diagonal strides on a DM) come precomputed from the host.
+struct Marg {
+  int32_t k, nt;
+  int64_t kstride[32];  // kept mode t: stride of its digit in the state
+  int64_t tstride[32];  // traced mode u (increasing mode order)
+};
+
 template <typename T>
 __global__ void __launch_bounds__(256)
-k_marginal(const C<T>* __restrict__ st, ModeList ml, int D, int mixed, int64_t NR, int64_t chunk, double* partial) {
+k_marginal(const C<T>* __restrict__ st, Marg mg, int D, int mixed, int64_t NR, int64_t chunk, double* partial) {
   __shared__ double sh[8];
-  const int n = ml.n, k = ml.k;
   int64_t offA = 0;
   {
     int64_t o = blockIdx.x;
-    for (int t = k - 1; t >= 0; --t) {
-      const int a = (int)(o % D);
+    for (int t = mg.k - 1; t >= 0; --t) {
+      offA += (o % D) * mg.kstride[t];
       o /= D;
-      const int m = ml.kept[t];
-      offA += a * (mixed ? dm_ket_stride(n, D, m) * (D + 1) : ket_stride(n, D, m));
     }
   }
   double v = 0.0;
@@ -49,11 +53,9 @@ k_marginal(const C<T>* __restrict__ st, ModeList ml, int D, int mixed, int64_t N
   const int64_t r1 = (r0 + chunk < NR) ? r0 + chunk : NR;
   for (int64_t r = r0 + threadIdx.x; r < r1; r += blockDim.x) {
     int64_t rr = r, off = 0;
-    for (int u = n - k - 1; u >= 0; --u) {
-      const int dgt = (int)(rr % D);
+    for (int u = mg.nt - 1; u >= 0; --u) {
+      off += (rr % D) * mg.tstride[u];
       rr /= D;
-      const int m = ml.traced[u];
-      off += dgt * (mixed ? dm_ket_stride(n, D, m) * (D + 1) : ket_stride(n, D, m));
     }
     const C<T> x = st[offA + off];
     v += mixed ? (double)x.re : (double)x.re * x.re + (double)x.im * x.im;
@@ -146,13 +148,15 @@ k_homodyne_pdf(const C<T>* __restrict__ rho, int D, double phi, double q_mag, in
 }
 
 // Embedding of a prepared k-mode state: out = rest (x) sigma with sigma's t-th
-// mode placed on register mode pos... modes[t] and the complement's modes in
-// increasing order (the product state of prepare_multimode after its axis
-// permutation).  DM layouts throughout; for a ket (k == n) out is the
-// permuted ket.
+// mode on register mode modes[t] and the complement's modes in increasing
+// order (the product state of prepare_multimode after its axis permutation).
+// DM layouts throughout; for a ket (k == n) out is the permuted ket.  Per
+// register mode the host supplies the strides of its bra and ket digits in
+// sigma (or in rest, the other factor's strides being zero).
 struct Embed {
-  int32_t n, k, nr, mixed;
-  int32_t slot[32];  // for register mode m: t >= 0 if sigma's axis t, else -(rank in the complement) - 1
+  int32_t n, nr, mixed, pad;
+  int64_t sb[32], sk[32];  // strides in sigma of register mode m's bra / ket digit
+  int64_t rb[32], rk[32];  // strides in rest
 };
 
 template <typename T>
@@ -162,22 +166,15 @@ k_embed(const C<T>* __restrict__ rest, const C<T>* __restrict__ sigma, C<T>* __r
   GRID_STRIDE(e, N) {
     int64_t r = e, si = 0, ri = 0;
     for (int m = em.n - 1; m >= 0; --m) {
-      int i, j = 0;
+      int j = 0;
       if (em.mixed) {
         j = (int)(r % D);
         r /= D;
       }
-      i = (int)(r % D);
+      const int i = (int)(r % D);
       r /= D;
-      const int sl = em.slot[m];
-      if (sl >= 0) {
-        const int t = sl;
-        si += em.mixed ? i * ipow(D, 2 * (em.k - 1 - t) + 1) + j * ipow(D, 2 * (em.k - 1 - t))
-                       : i * ipow(D, em.k - 1 - t);
-      } else {
-        const int u = -sl - 1;
-        ri += i * ipow(D, 2 * (em.nr - 1 - u) + 1) + j * ipow(D, 2 * (em.nr - 1 - u));
-      }
+      si += i * em.sb[m] + j * em.sk[m];
+      ri += i * em.rb[m] + j * em.rk[m];
     }
     const C<T> s = sigma[si];
     out[e] = em.nr > 0 ? cmul(rest[ri], s) : s;
@@ -197,16 +194,22 @@ extern "C" int sfamd_marginal_probs(const void* state, int n, int D, int mixed, 
   CHECK_ARG(dtype == SFAMD_C64 || dtype == SFAMD_C128);
   ModeList ml;
   RET_IF(make_modelist(n, modes, k, &ml));
+  Marg mg{};
+  mg.k = k;
+  mg.nt = n - k;
+  auto dstride = [&](int m) { return mixed ? dm_ket_stride(n, D, m) * (D + 1) : ket_stride(n, D, m); };
+  for (int t = 0; t < k; ++t) mg.kstride[t] = dstride(ml.kept[t]);
+  for (int u = 0; u < n - k; ++u) mg.tstride[u] = dstride(ml.traced[u]);
   hipStream_t s = (hipStream_t)stream;
   const int64_t NO = ipow(D, k), NR = ipow(D, n - k);
   const int64_t chunks = marginal_chunks(NR), chunk = (NR + chunks - 1) / chunks;
   CHECK_ARG(NO < (int64_t)1 << 31);
   dim3 grid((unsigned)NO, (unsigned)chunks);
   if (dtype == SFAMD_C64)
-    hipLaunchKernelGGL(k_marginal<float>, grid, dim3(kBlock), 0, s, CP(float, state), ml, D, mixed, NR, chunk,
+    hipLaunchKernelGGL(k_marginal<float>, grid, dim3(kBlock), 0, s, CP(float, state), mg, D, mixed, NR, chunk,
                        partial);
   else
-    hipLaunchKernelGGL(k_marginal<double>, grid, dim3(kBlock), 0, s, CP(double, state), ml, D, mixed, NR, chunk,
+    hipLaunchKernelGGL(k_marginal<double>, grid, dim3(kBlock), 0, s, CP(double, state), mg, D, mixed, NR, chunk,
                        partial);
   HIP_RET(hipGetLastError());
   hipLaunchKernelGGL(k_marginal_final, dim3(grid_for(NO)), dim3(kBlock), 0, s, partial, out_dev, NO, (int)chunks);
@@ -272,18 +275,27 @@ extern "C" int sfamd_embed_modes(const void* rest, const void* sigma, void* out,
   CHECK_ARG(rest || k == n);
   Embed em{};
   em.n = n;
-  em.k = k;
   em.nr = n - k;
   em.mixed = mixed ? 1 : 0;
-  for (int m = 0; m < n; ++m) em.slot[m] = -1000;
+  int slot[32];
+  for (int m = 0; m < n; ++m) slot[m] = -1000;
   for (int t = 0; t < k; ++t) {
     const int m = modes[t];
-    CHECK_ARG(m >= 0 && m < n && em.slot[m] == -1000);
-    em.slot[m] = t;
+    CHECK_ARG(m >= 0 && m < n && slot[m] == -1000);
+    slot[m] = t;
   }
   int u = 0;
-  for (int m = 0; m < n; ++m)
-    if (em.slot[m] == -1000) em.slot[m] = -(u++) - 1;
+  for (int m = 0; m < n; ++m) {
+    if (slot[m] >= 0) {
+      const int t = slot[m];
+      em.sb[m] = mixed ? ipow(D, 2 * (k - 1 - t) + 1) : ipow(D, k - 1 - t);
+      em.sk[m] = mixed ? ipow(D, 2 * (k - 1 - t)) : 0;
+    } else {
+      em.rb[m] = ipow(D, 2 * (em.nr - 1 - u) + 1);
+      em.rk[m] = ipow(D, 2 * (em.nr - 1 - u));
+      ++u;
+    }
+  }
   hipStream_t s = (hipStream_t)stream;
   const int64_t N = ipow(D, mixed ? 2 * n : n);
   if (dtype == SFAMD_C64)
