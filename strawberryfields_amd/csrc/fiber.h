@@ -18,11 +18,13 @@ int axis_dense_fib(const void* in, void* out, int64_t N, int D, int64_t B, const
 int axis_dense_adj_fib(const void* psi, const void* lam_out, void* lam_in, int64_t N, int D, int64_t B, const void* M,
                        const void* dM0, const void* dM1, int conj, int parity, double* partial, int nwg, int zp1,
                        int dtype, hipStream_t s);
+// diff = 1: difference-conserving blocks (two-mode squeezing) instead of
+// photon-number blocks
 int pair_bs_fib(const void* in, void* out, int64_t N, int D, int64_t sp, int64_t sq, const void* blocks, int conj,
-                int dtype, hipStream_t s);
+                int dtype, hipStream_t s, int diff = 0);
 int pair_bs_adj_fib(const void* psi, const void* lam_out, void* lam_in, int64_t N, int D, int64_t sp, int64_t sq,
                     const void* blocks, const void* dblocks, int conj, int wa, int wb, double* partial, int nwg,
-                    int zp1, int dtype, hipStream_t s);
+                    int zp1, int dtype, hipStream_t s, int diff = 0);
 int loss_fib(const void* in, void* out, int64_t N, int D, int64_t B, const void* coef, int dtype, hipStream_t s);
 int loss_adj_fib(const void* psi, const void* lam_out, void* lam_in, int64_t N, int D, int64_t B, const void* coef,
                  const void* dcoef, double* partial, int nwg, int zp1, int dtype, hipStream_t s);

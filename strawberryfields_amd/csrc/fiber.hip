@@ -140,7 +140,7 @@ k_axis_dense_adj_fib(const C<T>* __restrict__ psi, const C<T>* __restrict__ lo_,
 template <typename T, int D>
 __global__ void __launch_bounds__(256)
 k_pair_bs_fib(const C<T>* __restrict__ in, C<T>* __restrict__ out, int64_t NF, Geo2 g, int64_t sp, int64_t sq,
-              const C<T>* __restrict__ blk, int conj) {
+              const C<T>* __restrict__ blk, int conj, int diff) {
   constexpr int NSB = 2 * D - 1;
   const int64_t ntask = (NF + kBlock - 1) / kBlock * NSB;
   for (int64_t task = blockIdx.x; task < ntask; task += gridDim.x) {
@@ -150,8 +150,10 @@ k_pair_bs_fib(const C<T>* __restrict__ in, C<T>* __restrict__ out, int64_t NF, G
     const int lo = bs_lo(s, D);
     const int c = bs_hi(s, D) - lo + 1;
     const C<T>* bb = blk + bs_block_off(s, D);
-    const int64_t base = fiber_base2(f, g, D) + lo * sp + (s - lo) * sq;
-    const int64_t step = sp - sq;
+    // photon-number blocks: amplitudes (N, s - N); difference blocks (diff, two-mode
+    // squeezing): (N, N - d), d = s - (D - 1)
+    const int64_t base = fiber_base2(f, g, D) + lo * sp + (int64_t)(diff ? lo - (s - (D - 1)) : s - lo) * sq;
+    const int64_t step = diff ? sp + sq : sp - sq;
     C<T> x[D];
 #pragma unroll
     for (int t = 0; t < D; ++t)
@@ -177,7 +179,7 @@ template <typename T, int D>
 __global__ void __launch_bounds__(256)
 k_pair_bs_adj_fib(const C<T>* __restrict__ psi, const C<T>* __restrict__ lo_, C<T>* __restrict__ li, int64_t NF,
                   Geo2 g, int64_t sp, int64_t sq, const C<T>* __restrict__ blk, const C<T>* __restrict__ dblk,
-                  int conj, int wa, int wb, double* partial, int nwg, int zp1) {
+                  int conj, int wa, int wb, int diff, double* partial, int nwg, int zp1) {
   constexpr int NSB = 2 * D - 1;
   const T sg = conj ? (T)-1 : (T)1;
   const int64_t ntask = (NF + kBlock - 1) / kBlock * NSB;
@@ -190,8 +192,10 @@ k_pair_bs_adj_fib(const C<T>* __restrict__ psi, const C<T>* __restrict__ lo_, C<
     const int c = bs_hi(s, D) - lo + 1;
     const C<T>* bb = blk + bs_block_off(s, D);
     const C<T>* db = dblk + bs_block_off(s, D);
-    const int64_t base = fiber_base2(f, g, D) + lo * sp + (s - lo) * sq;
-    const int64_t step = sp - sq;
+    // photon-number blocks: amplitudes (N, s - N); difference blocks (diff, two-mode
+    // squeezing): (N, N - d), d = s - (D - 1)
+    const int64_t base = fiber_base2(f, g, D) + lo * sp + (int64_t)(diff ? lo - (s - (D - 1)) : s - lo) * sq;
+    const int64_t step = diff ? sp + sq : sp - sq;
     C<T> x[D], l[D];
 #pragma unroll
     for (int t = 0; t < D; ++t)
@@ -403,7 +407,7 @@ int axis_dense_adj_fib(const void* psi, const void* lam_out, void* lam_in, int64
 }
 
 int pair_bs_fib(const void* in, void* out, int64_t N, int D, int64_t sp, int64_t sq, const void* blocks, int conj,
-                int dtype, hipStream_t s) {
+                int dtype, hipStream_t s, int diff) {
   const int64_t NF = N / ((int64_t)D * D);
   const Geo2 g = geo2(sp, sq, N, D);
   const int grid = task_grid(NF, D);
@@ -412,10 +416,10 @@ int pair_bs_fib(const void* in, void* out, int64_t N, int D, int64_t sp, int64_t
   case d:                                                                                                          \
     if (dtype == SFAMD_C64)                                                                                        \
       hipLaunchKernelGGL((k_pair_bs_fib<float, d>), dim3(grid), dim3(kBlock), 0, s, CP(float, in), MP(float, out), \
-                         NF, g, sp, sq, CP(float, blocks), conj);                                                  \
+                         NF, g, sp, sq, CP(float, blocks), conj, diff);                                            \
     else                                                                                                           \
       hipLaunchKernelGGL((k_pair_bs_fib<double, d>), dim3(grid), dim3(kBlock), 0, s, CP(double, in),             \
-                         MP(double, out), NF, g, sp, sq, CP(double, blocks), conj);                                \
+                         MP(double, out), NF, g, sp, sq, CP(double, blocks), conj, diff);                          \
     break;
     SFAMD_FIB_D(CASE)
 #undef CASE
@@ -427,7 +431,7 @@ int pair_bs_fib(const void* in, void* out, int64_t N, int D, int64_t sp, int64_t
 
 int pair_bs_adj_fib(const void* psi, const void* lam_out, void* lam_in, int64_t N, int D, int64_t sp, int64_t sq,
                     const void* blocks, const void* dblocks, int conj, int wa, int wb, double* partial, int nwg,
-                    int zp1, int dtype, hipStream_t s) {
+                    int zp1, int dtype, hipStream_t s, int diff) {
   const int64_t NF = N / ((int64_t)D * D);
   const Geo2 g = geo2(sp, sq, N, D);
   switch (D) {
@@ -436,11 +440,11 @@ int pair_bs_adj_fib(const void* psi, const void* lam_out, void* lam_in, int64_t 
     if (dtype == SFAMD_C64)                                                                                          \
       hipLaunchKernelGGL((k_pair_bs_adj_fib<float, d>), dim3(nwg), dim3(kBlock), 0, s, CP(float, psi),             \
                          CP(float, lam_out), MP(float, lam_in), NF, g, sp, sq, CP(float, blocks), CP(float, dblocks), \
-                         conj, wa, wb, partial, nwg, zp1);                                                           \
+                         conj, wa, wb, diff, partial, nwg, zp1);                                                     \
     else                                                                                                             \
       hipLaunchKernelGGL((k_pair_bs_adj_fib<double, d>), dim3(nwg), dim3(kBlock), 0, s, CP(double, psi),           \
                          CP(double, lam_out), MP(double, lam_in), NF, g, sp, sq, CP(double, blocks),               \
-                         CP(double, dblocks), conj, wa, wb, partial, nwg, zp1);                                      \
+                         CP(double, dblocks), conj, wa, wb, diff, partial, nwg, zp1);                                \
     break;
     SFAMD_FIB_D(CASE)
 #undef CASE

@@ -123,10 +123,14 @@ int bs_adj_pass(const void* psi, const void* lo, void* li, int64_t N, int D, int
 int pair_pass(int kind, const void* in, void* out, int64_t N, int D, int64_t sp, int64_t sq, const void* g, int conj,
               int dtype, hipStream_t s) {
   if (kind == SFAMD_OP_TWO_MODE_SQUEEZE) {
-    // complex64: the beamsplitter's matrix-core and slab kernels on difference blocks
+    // the beamsplitter's per-fiber (matrix-core for complex64) and slab kernels on
+    // difference blocks; the element-wise S2 kernel otherwise
+    const bool fib = use_fib(D, sp < sq ? sp : sq);
     if (mfma_supported(D, dtype)) {
-      if (use_fib(D, sp < sq ? sp : sq)) return pair_bs_mfma(in, out, N, D, sp, sq, g, conj, s, 1);
+      if (fib) return pair_bs_mfma(in, out, N, D, sp, sq, g, conj, s, 1);
       if (use_slab(N, D, sp, sq, dtype, 0)) return pair_bs_slab(in, out, N, D, sp, sq, g, conj, dtype, s, 1);
+    } else if (fib) {
+      return pair_bs_fib(in, out, N, D, sp, sq, g, conj, dtype, s, 1);
     }
     return pair_s2(in, out, N, D, sp, sq, g, conj, dtype, s);
   }
@@ -137,11 +141,13 @@ int pair_adj_pass(int kind, const void* psi, const void* lo, void* li, int64_t N
                   const void* g, const void* dG, int conj, double* pb, int nwg, int zp1, int dtype, hipStream_t s) {
   if (kind == SFAMD_OP_TWO_MODE_SQUEEZE) {
     // d/dphi G[N][n] = i (N - n) G[N][n]: weights wa = -1, wb = 1
+    const bool fib = use_fib(D, sp < sq ? sp : sq);
     if (mfma_supported(D, dtype)) {
-      if (use_fib(D, sp < sq ? sp : sq))
-        return pair_bs_adj_mfma(psi, lo, li, N, D, sp, sq, g, dG, conj, -1, 1, pb, nwg, zp1, s, 1);
+      if (fib) return pair_bs_adj_mfma(psi, lo, li, N, D, sp, sq, g, dG, conj, -1, 1, pb, nwg, zp1, s, 1);
       if (use_slab(N, D, sp, sq, dtype, 1))
         return pair_bs_adj_slab(psi, lo, li, N, D, sp, sq, g, dG, conj, -1, 1, pb, nwg, zp1, dtype, s, 1);
+    } else if (fib) {
+      return pair_bs_adj_fib(psi, lo, li, N, D, sp, sq, g, dG, conj, -1, 1, pb, nwg, zp1, dtype, s, 1);
     }
     return pair_s2_adj(psi, lo, li, N, D, sp, sq, g, dG, conj, pb, nwg, zp1, dtype, s);
   }
