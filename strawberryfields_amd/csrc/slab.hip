@@ -570,18 +570,23 @@ __global__ void __launch_bounds__(256) k_slab_mfma(const Cf* __restrict__ in, Cf
   }
 }
 
-// one sweep of a block's fibers with one operator: kind 0/1 accumulate the
-// gradient Re conj(lam) (A x) into g; kind 2 writes lam_in = A lam in place
-template <int MB, int D, class P, typename F>
-__device__ __forceinline__ void slab_sweep_adj(const Cf* shx, Cf* shl, const P& pol, int b, int n, const Tile& t,
-                                               const View& v, F entry, int kind, double& g) {
+// adjoint of one block in one sweep over its fibers: the gradient operators
+// (dB/dp0, dB/dp1) act on the forward fiber x and accumulate Re conj(lam) (A x);
+// then lam_in = B^H lam overwrites lam in place (each fiber belongs to this wave
+// alone).  The three operators live in registers: the kernel runs one 512-thread
+// workgroup per CU (its LDS), so registers are not what bounds occupancy.
+template <int MB, int D, class P>
+__device__ __forceinline__ void slab_block_adj(const Cf* shx, Cf* shl, const P& pol, int b, int n, const Tile& t,
+                                               const View& v, double& g0, double& g1) {
   constexpr int Q = Blk<MB>::Q;
-  float A[MB][Blk<MB>::NS];
-  sfamd::mf::load_op<MB>(entry, A);
+  float A0[MB][Blk<MB>::NS], A1[MB][Blk<MB>::NS], AH[MB][Blk<MB>::NS];
+  sfamd::mf::load_op<MB>([&](int i, int k) { return pol.dentry0(b, n, i, k); }, A0);
+  if (P::kTwoParams) sfamd::mf::load_op<MB>([&](int i, int k) { return pol.dentry1(b, n, i, k); }, A1);
+  sfamd::mf::load_op<MB>([&](int i, int k) { return pol.entry_h(b, n, i, k); }, AH);
   const int per = pol.per_block(t, v);
   const int lane = threadIdx.x & 63;
   const int cl = lane & 15, gq = (lane >> 4) * Q;
-  float sum = 0.f;
+  float s0 = 0.f, s1 = 0.f;
   for (int fb = 0; fb * 16 < per; ++fb) {
     const int phi = fb * 16 + cl;
     const bool valid = phi < per;
@@ -593,33 +598,26 @@ __device__ __forceinline__ void slab_sweep_adj(const Cf* shx, Cf* shl, const P& 
     for (int q = 0; q < Q; ++q) {
       const bool in_range = valid && gq + q < n;
       l[q] = in_range ? shl[pos0 + q * step] : mk<float>(0.f, 0.f);
-      if (kind != 2) x[q] = in_range ? shx[pos0 + q * step] : mk<float>(0.f, 0.f);
+      x[q] = in_range ? shx[pos0 + q * step] : mk<float>(0.f, 0.f);
     }
     f4 acc[MB];
-    if (kind == 2) {
-      sfamd::mf::apply_op<MB>(A, l, acc);
-      if (valid) {
+    sfamd::mf::apply_op<MB>(A0, x, acc);
 #pragma unroll
-        for (int q = 0; q < Q; ++q)
-          if (gq + q < n) shl[pos0 + q * step] = sfamd::mf::acc_at<MB>(acc, q);
-      }
-    } else {
-      sfamd::mf::apply_op<MB>(A, x, acc);
+    for (int q = 0; q < Q; ++q) s0 += re_conj_mul(l[q], sfamd::mf::acc_at<MB>(acc, q));
+    if (P::kTwoParams) {
+      sfamd::mf::apply_op<MB>(A1, x, acc);
 #pragma unroll
-      for (int q = 0; q < Q; ++q) sum += re_conj_mul(l[q], sfamd::mf::acc_at<MB>(acc, q));
+      for (int q = 0; q < Q; ++q) s1 += re_conj_mul(l[q], sfamd::mf::acc_at<MB>(acc, q));
+    }
+    sfamd::mf::apply_op<MB>(AH, l, acc);
+    if (valid) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q)
+        if (gq + q < n) shl[pos0 + q * step] = sfamd::mf::acc_at<MB>(acc, q);
     }
   }
-  g += (double)sum;
-}
-
-// adjoint of one block: gradient sweeps first (they read lam), then lam_in
-template <int MB, int D, class P>
-__device__ __forceinline__ void slab_block_adj(const Cf* shx, Cf* shl, const P& pol, int b, int n, const Tile& t,
-                                               const View& v, double& g0, double& g1) {
-  slab_sweep_adj<MB, D>(shx, shl, pol, b, n, t, v, [&](int i, int k) { return pol.dentry0(b, n, i, k); }, 0, g0);
-  if (P::kTwoParams)
-    slab_sweep_adj<MB, D>(shx, shl, pol, b, n, t, v, [&](int i, int k) { return pol.dentry1(b, n, i, k); }, 1, g1);
-  slab_sweep_adj<MB, D>(shx, shl, pol, b, n, t, v, [&](int i, int k) { return pol.entry_h(b, n, i, k); }, 2, g0);
+  g0 += (double)s0;
+  g1 += (double)s1;
 }
 
 template <int D, class P>
