@@ -317,25 +317,32 @@ class Squeezed(Operation):
 
 
 class Interferometer(Operation):
-    """Passive linear interferometer U (ops.py:2556-2718), compiled into Rgate and
-    BSgate by the rectangular (Clements) mesh, as the fock compiler does."""
+    """Passive linear interferometer U (ops.py:2556-2718), compiled as the fock
+    compiler does: Rgate and BSgate for the rectangular, rectangular_phase_end
+    and triangular meshes, MZgate and Rgate for rectangular_symmetric.  The
+    compact meshes (rectangular_compact, triangular_compact, sun_compact) raise
+    NotImplementedError; an unknown mesh raises ValueError as in ops.py:2638-2651."""
 
     def __init__(self, U, mesh="rectangular", drop_identity=True, tol=1e-6):
+        from .interferometer import COMPACT_MESHES, MESHES
+
         U = np.asarray(U)
         super().__init__([U])
         self.ns = U.shape[0]
-        if mesh != "rectangular":
-            raise NotImplementedError("only the rectangular mesh is implemented")
+        if mesh not in MESHES + COMPACT_MESHES:
+            raise ValueError(f"Unknown mesh '{mesh}'. Mesh must be one of {set(MESHES + COMPACT_MESHES)}")
+        if mesh in COMPACT_MESHES:
+            raise NotImplementedError(f"mesh {mesh!r} is not implemented; use one of {MESHES}")
         self.mesh, self.drop_identity, self.tol = mesh, drop_identity, tol
 
     def decompose(self, reg):
         """[(gate, registers)] in application order."""
         from .interferometer import interferometer_gates
 
+        make = {"R": lambda p: Rgate(p[0]), "BS": lambda p: BSgate(p[0], p[1]), "MZ": lambda p: MZgate(p[0], p[1])}
         out = []
-        for name, modes, p in interferometer_gates(self.p[0], self.drop_identity, self.tol):
-            op = Rgate(p[0]) if name == "R" else BSgate(p[0], p[1])
-            out.append((op, [reg[m] for m in modes]))
+        for name, modes, p in interferometer_gates(self.p[0], self.drop_identity, self.tol, self.mesh):
+            out.append((make[name](p), [reg[m] for m in modes]))
         return out
 
     def apply(self, reg, backend, **kwargs):

@@ -1072,10 +1072,12 @@ def test_engine_collects_measurement_samples():
 
 @pytest.mark.parametrize("dtype", [C128, C64])
 @pytest.mark.parametrize("pure", [True, False])
-def test_interferometer_matches_exact_passive_action(dtype, pure):
-    """Interferometer(U) (rectangular mesh on the BS / R kernels) on a 4-mode
-    state with fewer than D photons equals the exact creation-operator action
-    of U, for the register in order and permuted."""
+@pytest.mark.parametrize("mesh", ["rectangular", "rectangular_phase_end", "rectangular_symmetric"])
+def test_interferometer_matches_exact_passive_action(dtype, pure, mesh):
+    """Interferometer(U) (rectangular meshes on the BS / R kernels, the
+    symmetric mesh on the MZ kernels) on a 4-mode state with fewer than D
+    photons equals the exact creation-operator action of U, for the register in
+    order and permuted."""
     from scipy.stats import unitary_group
 
     n, D = 4, 5
@@ -1090,7 +1092,7 @@ def test_interferometer_matches_exact_passive_action(dtype, pure):
         prog = frontend.Program(n)
         with prog.context as q:
             frontend.Ket(psi if pure else fo.mix(psi, n)) | q
-            frontend.Interferometer(U) | tuple(q[m] for m in order)
+            frontend.Interferometer(U, mesh=mesh) | tuple(q[m] for m in order)
         be = HIPFockBackend()
         res = frontend.Engine(be, backend_options={"cutoff_dim": D, "pure": pure, "dtype": dtype}).run(prog)
         # U acts on the register in `order`: permute U onto the mode axes

@@ -307,6 +307,12 @@ def _mode_unitary(gates, N):
         M = np.identity(N, dtype=complex)
         if name == "R":
             M[modes[0], modes[0]] = np.exp(1j * p[0])
+        elif name == "MZ":
+            # closed form of the Mach-Zehnder mode unitary (decompositions.py:391-396)
+            m, n = modes
+            si, co, e = np.sin(p[0] / 2), np.cos(p[0] / 2), np.exp(1j * p[1])
+            M[m, m], M[m, n], M[n, m], M[n, n] = si * e, co, co * e, -si
+            M[[m, m, n, n], [m, n, m, n]] *= 1j * np.exp(1j * p[0] / 2)
         else:
             m, n = modes
             c, s = np.cos(p[0]), np.sin(p[0])
@@ -336,6 +342,65 @@ def test_rectangular_mesh_reconstructs_the_unitary():
         interferometer_gates(np.ones((3, 3)))
 
 
+def test_other_meshes_follow_their_decomposition_identities():
+    """The decompositions behind the other meshes satisfy the identities the
+    reference's tests state for them (tests/frontend/test_decompositions.py:
+    225-330, 342-383), on Haar unitaries of size 2..20, the identity and the
+    anti-identity (zero and swap branches): phase-end U = D prod T; Mach-Zehnder
+    U = prod M^{-1} D prod M with phases in [0, 2 pi); symmetric U = D prod M;
+    triangular U = prod T^{-1} D.  The compiled gate lists of the
+    rectangular_phase_end and rectangular_symmetric meshes multiply back to U
+    (MZ checked against its closed-form matrix); the triangular mesh compiles as
+    the reference compiles it (R(phi), BS(theta, 0) per entry, which realises
+    D prod T, not U: the reference's Interferometer test leaves it out)."""
+    from scipy.stats import unitary_group
+    from strawberryfields_amd import interferometer as it
+
+    def prod(mats, start):
+        q = start
+        for m in mats:
+            q = m @ q
+        return q
+
+    cases = [unitary_group.rvs(N, random_state=30 + N) for N in (2, 3, 4, 5, 8, 20)]
+    cases += [np.identity(20), np.identity(20)[::-1], np.identity(3)[::-1], np.eye(4)[[2, 0, 3, 1]]]
+    for U in cases:
+        N = len(U)
+        tl, d, none = it.rectangular_phase_end(U)
+        assert none is None
+        assert np.abs(np.diag(d) @ prod([it._t(*t) for t in tl], np.identity(N)) - U).max() < 1e-13
+        r, d, l = it.rectangular_mz(U)
+        for t in r + l:
+            assert 0 <= t[2] < 2 * np.pi and 0 <= t[3] < 2 * np.pi
+        q = prod([it.mach_zehnder(*t).conj().T for t in reversed(l)],
+                 np.diag(d) @ prod([it.mach_zehnder(*t) for t in r], np.identity(N)))
+        assert np.abs(q - U).max() < 1e-12
+        tl, d, _ = it.rectangular_symmetric(U)
+        for t in tl:
+            assert 0 <= t[2] < 2 * np.pi and 0 <= t[3] < 2 * np.pi
+        assert np.abs(np.diag(d) @ prod([it.mach_zehnder(*t) for t in tl], np.identity(N)) - U).max() < 1e-12
+        tl, d, _ = it.triangular(U)
+        assert len(tl) == N * (N - 1) // 2
+        assert np.abs(prod([it._t_inv(*t) for t in tl], np.diag(d)) - U).max() < 1e-13
+        if N <= 8 and not np.allclose(U, np.identity(N)):
+            for mesh in ("rectangular_phase_end", "rectangular_symmetric"):
+                g = it.interferometer_gates(U, mesh=mesh)
+                assert np.abs(_mode_unitary(g, N) - U).max() < 1e-12, mesh
+                assert np.abs(_mode_unitary(it.interferometer_gates(U, False, mesh=mesh), N) - U).max() < 1e-12
+            assert all(x[0] == "MZ" for x in g[: N * (N - 1) // 2])
+            g = it.interferometer_gates(U, False, mesh="triangular")
+            realised = np.diag(d) @ prod([it._t(*t) for t in tl], np.identity(N))
+            assert np.abs(_mode_unitary(g, N) - realised).max() < 1e-12
+    for mesh in it.MESHES:
+        assert it.interferometer_gates(np.eye(4), mesh=mesh) == []
+    with pytest.raises(NotImplementedError):
+        it.interferometer_gates(cases[0], mesh="sun_compact")
+    with pytest.raises(ValueError):
+        it.interferometer_gates(cases[0], mesh="hexagonal")
+    with pytest.raises(ValueError):
+        it.rectangular_symmetric(np.ones((3, 3)))
+
+
 def test_interferometer_frontend_issues_backend_calls():
     from scipy.stats import unitary_group
 
@@ -350,5 +415,13 @@ def test_interferometer_frontend_issues_backend_calls():
     # register order is respected: the mesh acts on (q2, q0, q1)
     regs = {c[-1] for c in calls if c[0] == "rotation"}
     assert regs <= {0, 1, 2}
+    prog = frontend.Program(3)
+    with prog.context as q:
+        frontend.Interferometer(U, mesh="rectangular_symmetric") | q
+    be = RecordingBackend()
+    frontend.Engine(be, backend_options={"cutoff_dim": 3}).run(prog)
+    assert {c[0] for c in be.calls[1:]} <= {"mzgate", "rotation"} and be.calls[1][0] == "mzgate"
     with pytest.raises(NotImplementedError):
-        frontend.Interferometer(U, mesh="triangular")
+        frontend.Interferometer(U, mesh="rectangular_compact")
+    with pytest.raises(ValueError):
+        frontend.Interferometer(U, mesh="hexagonal")

@@ -686,5 +686,6 @@ def test_passive_fock_action_known_answers():
     for occ in itertools.product(range(D), repeat=3):
         if sum(occ) < D:
             psi3[occ] = rng.normal() + 1j * rng.normal()
-    st, _ = fo.run_ops(interferometer_gates(U), 3, D, state=psi3)
-    assert np.abs(fo.passive_fock_action(U, psi3, 3, D) - st).max() < 1e-13
+    for mesh in ("rectangular", "rectangular_phase_end", "rectangular_symmetric"):
+        st, _ = fo.run_ops(interferometer_gates(U, mesh=mesh), 3, D, state=psi3)
+        assert np.abs(fo.passive_fock_action(U, psi3, 3, D) - st).max() < 1e-13, mesh
