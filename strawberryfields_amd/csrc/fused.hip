@@ -249,11 +249,33 @@ __device__ __forceinline__ void adjoint_gate(const sfamd::StageGate& g, const C<
   }
 }
 
+// One DPP lane move of a double (two 32-bit moves); lanes of rows outside
+// RowMask receive 0.
+template <int Ctrl, int RowMask>
+__device__ __forceinline__ double dpp_move(double v) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, Ctrl, RowMask, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), Ctrl, RowMask, 0xf, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Sum over the 64 lanes of a wave by DPP moves, which stay in the VALU instead
+// of a round trip through the LDS crossbar per step as shuffles do: swaps
+// within pairs and quads, then the half-row and row mirrors leave every lane of
+// a 16-lane row with the row's sum; row broadcast 15 adds row 0 into row 1 and
+// row 2 into row 3, row broadcast 31 adds rows 0-1 into row 3.  The total is
+// valid in lane 63 (the summation order is fixed, so results are
+// deterministic).
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  v += dpp_move<0xB1, 0xF>(v);   // quad_perm [1, 0, 3, 2]
+  v += dpp_move<0x4E, 0xF>(v);   // quad_perm [2, 3, 0, 1]
+  v += dpp_move<0x141, 0xF>(v);  // row_half_mirror
+  v += dpp_move<0x140, 0xF>(v);  // row_mirror
+  v += dpp_move<0x142, 0xA>(v);  // row_bcast:15 into rows 1, 3
+  v += dpp_move<0x143, 0xC>(v);  // row_bcast:31 into rows 2, 3
   return v;
 }
+constexpr int kWaveSumLane = 63;
 
 // global offset of tile element e (relative to the tile base)
 template <int D, int K>
@@ -332,9 +354,9 @@ __global__ void __launch_bounds__(NT) k_stage_adj(const C<T>* __restrict__ psi_i
                                                         sfamd::StageArgs a, double* __restrict__ partial, int nwg) {
   constexpr int DK = Pow<D, K>::v;
   extern __shared__ __align__(16) unsigned char smem[];
-  // layout: [ngates + 1 psi tiles][2 lam tiles][matrices + derivatives][grad slots [ngates][2][kWaves]]
+  // layout: [ngates psi tiles (each gate's input)][2 lam tiles][matrices + derivatives][grad slots [ngates][2][kWaves]]
   C<T>* tiles = reinterpret_cast<C<T>*>(smem);
-  C<T>* lamA = tiles + (a.ngates + 1) * DK;
+  C<T>* lamA = tiles + a.ngates * DK;
   C<T>* lamB = lamA + DK;
   C<T>* mats = lamB + DK;
   double* gsl = reinterpret_cast<double*>(mats + a.mat_adj);
@@ -351,8 +373,9 @@ __global__ void __launch_bounds__(NT) k_stage_adj(const C<T>* __restrict__ psi_i
       lamA[e] = lam_out[off];
     }
     __syncthreads();
-    // recompute the stage forward, keeping every intermediate tile
-    for (int gi = 0; gi < a.ngates; ++gi) {
+    // recompute the stage forward, keeping every intermediate tile; the adjoint
+    // of gate gi reads only its input tile, so the last gate's output is not formed
+    for (int gi = 0; gi < a.ngates - 1; ++gi) {
       apply_gate<T, D, K, NT>(a.g[gi], mats + a.g[gi].amoff, tiles + gi * DK, tiles + (gi + 1) * DK);
       __syncthreads();
     }
@@ -361,11 +384,13 @@ __global__ void __launch_bounds__(NT) k_stage_adj(const C<T>* __restrict__ psi_i
     for (int gi = a.ngates - 1; gi >= 0; --gi) {
       double g0 = 0.0, g1 = 0.0;
       adjoint_gate<T, D, K, NT>(a.g[gi], mats + a.g[gi].amoff, tiles + gi * DK, lo_, li, g0, g1);
+      // R and K have one parameter: their second slot stays at its cleared zero
+      const bool two = a.g[gi].kind != SFAMD_OP_ROTATION && a.g[gi].kind != SFAMD_OP_KERR;
       g0 = wave_sum(g0);
-      g1 = wave_sum(g1);
-      if (lane == 0) {  // each wave owns its slot: deterministic, no atomics
+      if (two) g1 = wave_sum(g1);
+      if (lane == kWaveSumLane) {  // each wave owns its slot: deterministic, no atomics
         gsl[(gi * 2 + 0) * kWaves + wave] += g0;
-        gsl[(gi * 2 + 1) * kWaves + wave] += g1;
+        if (two) gsl[(gi * 2 + 1) * kWaves + wave] += g1;
       }
       __syncthreads();
       C<T>* tmp = lo_;
@@ -516,7 +541,7 @@ int64_t stage_lds_fwd(int D, int K, int mat_fwd, int dtype) {
 
 int64_t stage_lds_adj(int D, int K, int ngates, int mat_adj, int dtype) {
   const int64_t esz = dtype == SFAMD_C64 ? 8 : 16;
-  return ((ngates + 3) * ipow(D, K) + mat_adj) * esz + (int64_t)ngates * 2 * (sfamd::stage_threads((int)ipow(D, K)) / 64) * 8;
+  return ((ngates + 2) * ipow(D, K) + mat_adj) * esz + (int64_t)ngates * 2 * (sfamd::stage_threads((int)ipow(D, K)) / 64) * 8;
 }
 
 int stage_fwd(int D, int K, const void* in, void* out, const void* table, const StageArgs& a, int grid, int dtype,

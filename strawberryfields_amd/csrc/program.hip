@@ -337,12 +337,12 @@ unsigned op_mask(const OpPlan& op) {
 // ops of `rem` (program order) absorbable into a stage on `mask`: an op joins
 // if its modes lie in the mask, no earlier op left outside the stage touches
 // one of them (gates on disjoint modes commute) and the adjoint kernel's LDS
-// (one tile per gate plus three, and every gate's matrix + derivatives) fits
+// (one tile per gate plus two, and every gate's matrix + derivatives) fits
 void absorb(const std::vector<OpPlan>& ops, const std::vector<int>& rem, unsigned mask, int D, int64_t tile_bytes,
             int64_t esz, std::vector<int>* got) {
   got->clear();
   unsigned blocked = 0;
-  int64_t used = 3 * tile_bytes;
+  int64_t used = 2 * tile_bytes;
   for (int i : rem) {
     const OpPlan& op = ops[i];
     const unsigned om = op_mask(op);
