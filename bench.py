@@ -203,7 +203,9 @@ def run_c5(args):
 
 def run_c2(args):
     """4-mode pure ket at cutoff 15 (50,625 amplitudes), 8 layers (256 ops),
-    forward only in complex64 and complex128: evals/s, norm and fock_prob; the
+    forward only in complex64 and complex128: evals/s launched eagerly (one
+    library call per forward) and replayed from a HIP graph, norm and
+    fock_prob; the
     CPU leg times the oracle's faithful fock-backend loop and also yields the
     complex128 reference state the GPU results are compared with."""
     import numpy as np
@@ -236,9 +238,26 @@ def run_c2(args):
         t = (time.perf_counter() - t0) / reps
         probs = runtime.all_fock_probs(psi, n, D, False)
         kets[name] = psi.cpu().numpy()
+        # the same forward (table build + every step, plus a fidelity readout)
+        # replayed from one captured HIP graph, timed with events on its stream
+        ev = runtime.GradEvaluator(cc, vac, vac, use_graph=True, forward_only=True)
+        ev(params)
+        ev.stream.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(ev.stream)
+        for _ in range(reps):
+            ev(params)
+        e1.record(ev.stream)
+        e1.synchronize()
+        t_graph = e0.elapsed_time(e1) / 1e3 / reps
+        dg = float((ev.psi - psi).abs().max())
+        del ev
         out[name] = {
             "fwd_evals_per_s": round(1.0 / t, 1),
             "ms": round(t * 1e3, 4),
+            "fwd_evals_per_s_graph": round(1.0 / t_graph, 1),
+            "ms_graph": round(t_graph * 1e3, 4),
+            "graph_vs_eager_max_abs_diff": dg,
             "fused_local_modes": K,
             "steps": cc.nsteps,
             "norm": float(runtime.norm2(psi, n, D, False)),
