@@ -23,7 +23,7 @@ import numbers
 import numpy as np
 import torch
 
-from . import runtime, _lib
+from . import fockstates, runtime, _lib
 from .base import BaseFock, ModeMap
 from .states import FockStateHIP
 
@@ -264,9 +264,22 @@ class HIPFockBackend(BaseFock):
         self._prepare(U[:, 0].clone(), self._remap_modes(mode))
 
     def prepare_displaced_squeezed_state(self, r_d, phi_d, r_s, phi_s, mode):
-        Dm = runtime.gate_matrix("D", r_d, phi_d, self._D, self._dtype, self._device)
-        Sm = runtime.gate_matrix("S", r_s, phi_s, self._D, self._dtype, self._device)
-        self._prepare(Dm @ Sm[:, 0], self._remap_modes(mode))
+        """Amplitudes from the closed form (fockbackend/ops.py:420-446), not
+        D[:, :D] @ S[:D, 0], whose product misses the levels >= D."""
+        v = fockstates.displaced_squeezed(float(r_d), float(phi_d), float(r_s), float(phi_s), self._D)
+        self._prepare(torch.as_tensor(v), self._remap_modes(mode))
+
+    def prepare_gkp(self, state, epsilon, ampl_cutoff, representation="real", shape="square", mode=None):
+        """Square-lattice GKP state cos(theta/2)|0> + e^{-i phi} sin(theta/2)|1>
+        (fockbackend/backend.py:297-336, circuit.py:803-812); the complex
+        representation and other lattices raise NotImplementedError as there."""
+        if representation == "complex":
+            raise NotImplementedError("The complex description of GKP is not implemented")
+        if shape != "square":
+            raise NotImplementedError("Only square GKP are implemented for now")
+        theta, phi = state[0], state[1]
+        v = fockstates.square_gkp(float(theta), float(phi), float(epsilon), float(ampl_cutoff), self._D)
+        self._prepare(torch.as_tensor(v), self._remap_modes(mode))
 
     def prepare_thermal_state(self, nbar, mode):
         k = np.arange(self._D)
@@ -471,4 +484,4 @@ class HIPFockBackend(BaseFock):
     def _not_here(self, *args, **kwargs):
         raise NotImplementedError("not implemented by the HIP Fock backend")
 
-    thermal_loss = measure_threshold = prepare_gkp = _not_here
+    thermal_loss = measure_threshold = _not_here

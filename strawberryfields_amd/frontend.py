@@ -316,6 +316,71 @@ class Squeezed(Operation):
         backend.prepare_squeezed_state(par_evaluate(self.p[0]), par_evaluate(self.p[1]), *reg)
 
 
+class DisplacedSqueezed(Operation):
+    """D(r_d e^{i phi_d}) S(r_s e^{i phi_s})|0> (ops.py:687-758)."""
+
+    ns = 1
+
+    def __init__(self, r_d=0.0, phi_d=0.0, r_s=0.0, phi_s=0.0):
+        super().__init__([r_d, phi_d, r_s, phi_s])
+
+    def _apply(self, reg, backend, **kwargs):
+        p = [par_evaluate(x) for x in self.p]
+        backend.prepare_displaced_squeezed_state(p[0], p[1], p[2], p[3], *reg)
+
+
+class Thermal(Operation):
+    """Thermal state with mean photon number n (ops.py:1106-1135)."""
+
+    ns = 1
+
+    def __init__(self, n=0):
+        super().__init__([n])
+
+    def _apply(self, reg, backend, **kwargs):
+        backend.prepare_thermal_state(par_evaluate(self.p[0]), *reg)
+
+
+class Catstate(Operation):
+    """(|a> + e^{i pi p}|-a>) / N, a = a e^{i phi}, in the complex (Fock)
+    representation (ops.py:825-931): the ket is formed on the host from the
+    coherent-state amplitudes alpha^n / sqrt(n!) and N = sqrt(2 (1 + cos(pi p)
+    e^{-2|a|^2})) e^{|a|^2/2}, then prepared with prepare_ket_state."""
+
+    ns = 1
+
+    def __init__(self, a=0.0, phi=0.0, p=0, representation="complex", ampl_cutoff=1e-12, D=2):
+        super().__init__([a, phi, p, representation, ampl_cutoff, D])
+
+    def _apply(self, reg, backend, **kwargs):
+        import math
+
+        a, phi, p = par_evaluate(self.p[0]), par_evaluate(self.p[1]), self.p[2]
+        alpha = a * np.exp(1j * phi)
+        theta = np.pi * p
+        cutoff = backend.get_cutoff_dim()
+        t = np.exp(-0.5 * abs(alpha) ** 2)
+        N = t / np.sqrt(2 * (1 + np.cos(theta) * t**4))
+        l = np.arange(cutoff)
+        sq = np.array([math.exp(0.5 * math.lgamma(k + 1)) for k in l])
+        ket = (alpha**l / sq + np.exp(1j * theta) * (-alpha) ** l / sq) * N
+        backend.prepare_ket_state(ket, *reg)
+
+
+class GKP(Operation):
+    """Finite-energy square-lattice GKP state (ops.py:934-968)."""
+
+    ns = 1
+
+    def __init__(self, state=None, epsilon=0.2, ampl_cutoff=1e-12, representation="real", shape="square"):
+        if state is None:
+            state = [0, 0]
+        super().__init__([state, epsilon, ampl_cutoff, representation, shape])
+
+    def _apply(self, reg, backend, **kwargs):
+        backend.prepare_gkp(*self.p, mode=reg[0])
+
+
 class Interferometer(Operation):
     """Passive linear interferometer U (ops.py:2556-2718), compiled as the fock
     compiler does: Rgate and BSgate for the rectangular, rectangular_phase_end

@@ -1199,3 +1199,71 @@ def test_maximum_cutoff_tables_and_ops(dtype):
                 ref, rpure = fo.run_ops(ops, 2, 40, state=st2)
                 tol = (1e-7 if dtype == C128 else 2e-5) if name in ("BS", "S2") else TOL[dtype] * 10
                 assert np.abs(to_np(got.ket() if rpure else got.dm()) - ref).max() < tol, (name, modes, pure)
+
+
+# ------------------------------------------------------------------ single-mode preparations
+@pytest.mark.parametrize("dtype", [C128, C64])
+def test_single_mode_preparations_match_closed_forms(dtype):
+    """Coherent, squeezed, displaced squeezed, thermal, GKP and cat states
+    prepared by the backend: on one mode they equal the states built from the
+    ladder operators with matrix exponentials at cutoff 160 (then truncated;
+    the displaced squeezed state to the 5e-10 that the reference's formula
+    carries), the thermal closed form and the host GKP expansion (pinned on the
+    CPU against its operator definition); on mode 1 of a two-mode register the
+    reduced state of that mode is the prepared one and mode 0 stays vacuum."""
+    from scipy.linalg import expm
+
+    from strawberryfields_amd import fockstates as fs
+
+    D, L = 14, 160
+    a = np.diag(np.sqrt(np.arange(1, L)), 1)
+    vac = np.zeros(L)
+    vac[0] = 1
+
+    def dsq(rd, pd, rs, ps):
+        al, z = rd * np.exp(1j * pd), rs * np.exp(1j * ps)
+        return (expm(al * a.T - np.conj(al) * a) @ expm(0.5 * (np.conj(z) * a @ a - z * a.T @ a.T)) @ vac)[:D]
+
+    tol = TOL[dtype] * 10
+    cases = [
+        ("prepare_coherent_state", (0.7, 0.4), dsq(0.7, 0.4, 0, 0), tol),
+        ("prepare_squeezed_state", (0.5, -0.3), dsq(0, 0, 0.5, -0.3), tol),
+        ("prepare_displaced_squeezed_state", (0.6, 1.1, 0.3, 0.2), dsq(0.6, 1.1, 0.3, 0.2), max(tol, 5e-10)),
+        ("prepare_gkp", ([0.7, 0.3], 0.35, 1e-12), fs.square_gkp(0.7, 0.3, 0.35, 1e-12, D), tol),
+    ]
+    for method, args, want, t in cases:
+        be = HIPFockBackend()
+        be.begin_circuit(1, cutoff_dim=D, pure=True, dtype=dtype)
+        if method == "prepare_gkp":
+            be.prepare_gkp(*args, mode=0)
+        else:
+            getattr(be, method)(*args, 0)
+        got = to_np(be.state().ket())
+        assert np.abs(got - want).max() < t, method
+        be = HIPFockBackend()
+        be.begin_circuit(2, cutoff_dim=D, pure=True, dtype=dtype)
+        if method == "prepare_gkp":
+            be.prepare_gkp(*args, mode=1)
+        else:
+            getattr(be, method)(*args, 1)
+        st = be.state()
+        assert np.abs(to_np(st.reduced_dm(1)) - np.outer(want, want.conj())).max() < t, method
+        r0 = to_np(st.reduced_dm(0))
+        v0 = np.zeros((D, D))
+        v0[0, 0] = np.vdot(want, want).real
+        assert np.abs(r0 - v0).max() < t, method
+    be = HIPFockBackend()
+    be.begin_circuit(1, cutoff_dim=D, pure=True, dtype=dtype)
+    be.prepare_thermal_state(0.6, 0)
+    k = np.arange(D)
+    assert np.abs(to_np(be.state().dm()) - np.diag(0.6**k / 1.6 ** (k + 1))).max() < tol
+    with pytest.raises(NotImplementedError):
+        be.prepare_gkp([0, 0], 0.2, 1e-12, representation="complex", mode=0)
+    prog = frontend.Program(1)
+    with prog.context as q:
+        frontend.Catstate(0.9, 0.2, 0) | q[0]
+    res = frontend.Engine(HIPFockBackend(), backend_options={"cutoff_dim": D, "dtype": dtype}).run(prog)
+    al = 0.9 * np.exp(0.2j)
+    want = dsq(0.9, 0.2, 0, 0) + dsq(-0.9, 0.2, 0, 0)
+    want /= np.sqrt(2 * (1 + np.exp(-2 * abs(al) ** 2)))
+    assert np.abs(to_np(res.state.ket()) - want).max() < tol

@@ -149,10 +149,14 @@ class RecordingBackend:
 
     def begin_circuit(self, num_subsystems, **kw):
         self.calls.append(("begin", num_subsystems, kw))
+        self.cutoff = kw.get("cutoff_dim")
+
+    def get_cutoff_dim(self):
+        return self.cutoff
 
     def __getattr__(self, name):
-        def f(*args):
-            self.calls.append((name,) + args)
+        def f(*args, **kw):
+            self.calls.append((name,) + args + ((kw,) if kw else ()))
 
         return f
 
@@ -425,3 +429,85 @@ def test_interferometer_frontend_issues_backend_calls():
         frontend.Interferometer(U, mesh="rectangular_compact")
     with pytest.raises(ValueError):
         frontend.Interferometer(U, mesh="hexagonal")
+
+
+# ------------------------------------------------------------------ single-mode preparations
+def _ladder_expm(L):
+    from scipy.linalg import expm
+
+    a = np.diag(np.sqrt(np.arange(1, L)), 1)
+    ad = a.T
+
+    def Dop(al):
+        return expm(al * ad - np.conj(al) * a)
+
+    def Sop(z):
+        return expm(0.5 * (np.conj(z) * a @ a - z * ad @ ad))
+
+    vac = np.zeros(L)
+    vac[0] = 1
+    return Dop, Sop, vac
+
+
+def test_displaced_squeezed_and_gkp_expansions_match_the_operator_definitions():
+    """fockstates.displaced_squeezed equals D(a) S(z)|0> computed with matrix
+    exponentials of the ladder operators at cutoff 160 (then truncated), to
+    5e-10: the reference's formula adds 1e-10 under a square root
+    (fockbackend/ops.py:437), which the closed form keeps, so the agreement
+    stops near that level; its zero branches (coherent state, squeezed vacuum)
+    are exact.  The GKP expansion equals its definition (normalised sum over the
+    lattice teeth of displaced squeezed states, ops.py:518-596) built the same
+    operator way; theta = 0 / pi select the logical states, phi enters as e^{-i phi}."""
+    from strawberryfields_amd import fockstates as fs
+
+    Dop, Sop, vac = _ladder_expm(160)
+    for rd, pd, rs, ps in [(0.5, 0.3, 0.4, -0.7), (1.2, -2.0, 0.2, 1.1), (0.3, 0.0, -0.5, 0.4), (2.0, 1.0, 1.0, 0.0)]:
+        ex = (Dop(rd * np.exp(1j * pd)) @ Sop(rs * np.exp(1j * ps)) @ vac)[:30]
+        assert np.abs(ex - fs.displaced_squeezed(rd, pd, rs, ps, 30)).max() < 5e-10
+    for rd, pd, rs, ps in [(0.0, 0.0, 0.6, 0.2), (0.7, 0.4, 0.0, 0.0), (0.0, 0.0, 0.0, 0.0)]:
+        ex = (Dop(rd * np.exp(1j * pd)) @ Sop(rs * np.exp(1j * ps)) @ vac)[:30]
+        assert np.abs(ex - fs.displaced_squeezed(rd, pd, rs, ps, 30)).max() < 1e-14
+
+    eps, cut = 0.3, 1e-12
+    r = -0.5 * np.log(np.tanh(eps))
+    zmax = int(np.ceil(np.sqrt(-0.25 / np.pi * np.log(cut) / np.tanh(eps))))
+
+    def logical(mu):
+        k = np.zeros(160, complex)
+        for t in range(-zmax, zmax + 1):
+            w = np.exp(-0.5 * np.pi * np.tanh(eps) * (mu + 2 * t) ** 2)
+            k += w * (Dop(np.sqrt(np.pi / 2) * (2 * t + mu) / np.cosh(eps)) @ Sop(r) @ vac)
+        return k[:40]
+
+    k0, k1 = logical(0), logical(1)
+    for th, ph in [(0.8, 0.5), (0.0, 0.0), (np.pi, 0.0), (np.pi / 2, -1.1)]:
+        k = np.cos(th / 2) * k0 + np.sin(th / 2) * np.exp(-1j * ph) * k1
+        got = fs.square_gkp(th, ph, eps, cut, 40)
+        assert abs(np.linalg.norm(got) - 1) < 1e-14
+        assert np.abs(k / np.linalg.norm(k) - got).max() < 1e-9, (th, ph)
+    # the teeth of logical |0> come in pairs x, -x, and D(-x)S(r)|0> is the
+    # parity image of D(x)S(r)|0>, so |0>_gkp has even photon numbers only
+    g0 = fs.square_gkp(0.0, 0.0, eps, cut, 40)
+    assert np.abs(g0[1::2]).max() < 1e-12
+
+
+def test_preparation_ops_issue_their_backend_calls():
+    """DisplacedSqueezed, Thermal, GKP and Catstate reach the backend as in the
+    reference (ops.py:687-758, 825-968, 1106-1135); the cat-state ket is the
+    normalised coherent superposition with the reference's parity phase."""
+    prog = frontend.Program(2)
+    with prog.context as q:
+        frontend.DisplacedSqueezed(0.3, 0.1, 0.2, 0.4) | q[0]
+        frontend.Thermal(0.5) | q[1]
+        frontend.GKP([0.4, 0.2], epsilon=0.25) | q[0]
+        frontend.Catstate(0.8, 0.3, 1) | q[1]
+    be = RecordingBackend()
+    frontend.Engine(be, backend_options={"cutoff_dim": 12}).run(prog)
+    names = [c[0] for c in be.calls[1:]]
+    assert names == ["prepare_displaced_squeezed_state", "prepare_thermal_state", "prepare_gkp", "prepare_ket_state"]
+    ket = be.calls[-1][1]
+    Dop, _, vac = _ladder_expm(80)
+    al = 0.8 * np.exp(0.3j)
+    want = ((Dop(al) @ vac) - (Dop(-al) @ vac))[:12]
+    want /= np.sqrt(2 * (1 - np.exp(-2 * abs(al) ** 2)))
+    assert np.abs(ket - want).max() < 1e-13
