@@ -239,6 +239,32 @@ class MZgate(Gate):
         backend.mzgate(phi_in, phi_ex, *reg)
 
 
+class sMZgate(Operation):
+    """Symmetric Mach-Zehnder interferometer with internal phases phi_in,
+    phi_ex (ops.py:2015-2031); the fock compiler decomposes it, and so does this
+    mirror: BS(pi/4, pi/2), R(phi_ex - pi/2) on the second mode, R(phi_in - pi/2)
+    on the first, BS(pi/4, pi/2)."""
+
+    ns = 2
+
+    def __init__(self, phi_in, phi_ex):
+        super().__init__([phi_in, phi_ex])
+
+    def decompose(self, reg):
+        from .interferometer import smz_gates
+
+        out = []
+        for name, modes, p in smz_gates((0, 1), par_evaluate(self.p[0]), par_evaluate(self.p[1])):
+            op = Rgate(p[0]) if name == "R" else BSgate(p[0], p[1])
+            out.append((op, [reg[m] for m in modes]))
+        return out
+
+    def apply(self, reg, backend, **kwargs):
+        regs = [r if isinstance(r, RegRef) else RegRef(r) for r in reg]
+        for op, rr in self.decompose(regs):
+            op.apply(rr, backend, **kwargs)
+
+
 class CKgate(Gate):
     """Cross-Kerr interaction exp(i kappa n1 n2) (strawberryfields/ops.py:2219-2257)."""
 
@@ -383,9 +409,9 @@ class GKP(Operation):
 
 class Interferometer(Operation):
     """Passive linear interferometer U (ops.py:2556-2718), compiled as the fock
-    compiler does: Rgate and BSgate for the rectangular, rectangular_phase_end
-    and triangular meshes, MZgate and Rgate for rectangular_symmetric.  The
-    compact meshes (rectangular_compact, triangular_compact, sun_compact) raise
+    compiler does: Rgate and BSgate for the rectangular, rectangular_phase_end,
+    triangular and the two compact meshes (their sMZ blocks decomposed), MZgate
+    and Rgate for rectangular_symmetric.  sun_compact raises
     NotImplementedError; an unknown mesh raises ValueError as in ops.py:2638-2651."""
 
     def __init__(self, U, mesh="rectangular", drop_identity=True, tol=1e-6):

@@ -189,9 +189,201 @@ def triangular(U, tol=1e-11):
     return list(reversed(tlist)), np.diag(V), None
 
 
-MESHES = ("rectangular", "rectangular_phase_end", "rectangular_symmetric", "triangular")
-# meshes the reference also accepts, whose phase parametrisations are not built here
-COMPACT_MESHES = ("rectangular_compact", "triangular_compact", "sun_compact")
+# ---- compact meshes of symmetric Mach-Zehnder blocks (Bell & Walmsley,
+# APL Photonics 6, 070804 (2021), arXiv:2104.07561), as decompositions.py:636-905
+# applies them.  A block on modes (n, n + 1) is
+#   M(sigma, delta) = e^{i sigma} [[sin delta, cos delta], [cos delta, -sin delta]],
+# the symmetric MZ with internal phases sigma +- delta; the elimination works on
+# V = conj(U) and clears entries column by column with blocks applied from the
+# right (triangular) or alternately from the right and the left (rectangular).
+
+
+def _smz(n, sigma, delta, N):
+    M = np.identity(N, dtype=np.complex128)
+    e = np.exp(1j * sigma)
+    M[n, n], M[n, n + 1], M[n + 1, n], M[n + 1, n + 1] = e * np.sin(delta), e * np.cos(delta), e * np.cos(delta), -e * np.sin(delta)
+    return M
+
+
+def _phase(j, phi, N):
+    M = np.identity(N, dtype=np.complex128)
+    M[j, j] = np.exp(1j * phi)
+    return M
+
+
+def _check_unitary(U, rtol, atol):
+    if U.shape[0] != U.shape[1]:
+        raise ValueError("Matrix is not square")
+    if not np.allclose(U @ U.conj().T, np.eye(U.shape[0]), rtol=rtol, atol=atol):
+        raise ValueError("The input matrix is not unitary")
+
+
+def _delta(big, small):
+    """delta with tan(delta) = -|small| / |big| (arctan2 of the two moduli); a
+    zero entry `big` gives pi / 2."""
+    return 0.5 * np.pi if big == 0 else float(np.arctan2(-abs(small), abs(big)))
+
+
+def _final_phases(V, N, rtol, atol):
+    """Residual diagonal: zeta_0 = -arg V[0, 0], zeta_j = arg V[0, 0] - arg V[j, j]
+    after the earlier ones are applied; V must end as the identity."""
+    zetas = {}
+    zetas[0] = float(-np.angle(V[0, 0]))
+    V = V @ _phase(0, zetas[0], N)
+    for j in range(1, N):
+        zetas[j] = float(np.angle(V[0, 0]) - np.angle(V[j, j]))
+        V = V @ _phase(j, zetas[j], N)
+    if not np.allclose(V, np.eye(N), rtol=rtol, atol=atol):
+        raise RuntimeError("decomposition failed")
+    return zetas
+
+
+def triangular_compact(U, rtol=1e-12, atol=1e-12):
+    """Phases of the compact triangular (Reck) mesh (decompositions.py:672-737):
+    dict with m, phi_ins (mode -> phi), deltas and sigmas ((mode, layer) ->
+    value) and zetas (mode -> output phase)."""
+    U = np.asarray(U)
+    _check_unitary(U, rtol, atol)
+    V = U.conj()
+    N = U.shape[0]
+    out = {"m": N, "phi_ins": {}, "deltas": {}, "sigmas": {}, "zetas": {}}
+    for j in range(N - 1):
+        x, y = N - 1, j
+        phi = float(np.angle(V[x, y]) - np.angle(V[x, y + 1]))
+        out["phi_ins"][j] = phi
+        V = V @ _phase(j + 1, phi, N)
+        for k in range(j + 1):
+            n = j - k
+            delta = _delta(V[x, y], V[x, y + 1])
+            W = V @ _smz(n, 0, delta, N)
+            # y - 1 is -1 for the last block of a column: numpy's wrap-around to
+            # the last column, as in the reference (the output phases absorb it)
+            sigma = float(np.angle(W[x - 1, y - 1]) - np.angle(W[x - 1, y]))
+            out["deltas"][n, k], out["sigmas"][n, k] = delta, sigma
+            V = V @ _smz(n, sigma, delta, N)
+            x, y = x - 1, y - 1
+    out["zetas"] = _final_phases(V, N, rtol, atol)
+    return out
+
+
+def _rectangular_compact_init(U, rtol, atol):
+    """Phases of the compact rectangular (Clements) mesh before the residual
+    phases are moved to the edge (decompositions.py:740-826)."""
+    V = U.conj()
+    N = U.shape[0]
+    out = {"m": N, "phi_ins": {}, "deltas": {}, "sigmas": {}, "zetas": {}, "phi_outs": {}}
+    for j in range(N - 1):
+        if j % 2 == 0:
+            x, y = N - 1, j
+            phi = float(np.angle(V[x, y + 1]) - np.angle(V[x, y]))
+            V = V @ _phase(j, phi, N)
+            out["phi_ins"][j] = phi
+            for k in range(j + 1):
+                delta = _delta(V[x, y], V[x, y + 1])
+                n = j - k
+                W = V @ _smz(n, 0, delta, N)
+                sigma = float(np.angle(W[x - 1, y - 1]) - np.angle(W[x - 1, y]))
+                V = V @ _smz(n, sigma, delta, N)
+                out["deltas"][n, k], out["sigmas"][n, k] = delta, sigma
+                x, y = x - 1, y - 1
+        else:
+            x, y = N - j - 1, 0
+            phi = float(np.angle(V[x - 1, y]) - np.angle(V[x, y]))
+            V = _phase(x, phi, N) @ V
+            out["phi_outs"][x] = phi
+            for k in range(j + 1):
+                delta = 0.5 * np.pi if V[x, y] == 0.0 else float(np.arctan2(abs(V[x - 1, y]), abs(V[x, y])))
+                W = _smz(x - 1, 0, delta, N) @ V
+                n = N + k - j - 2
+                sigma = float(np.angle(W[x + 1, y + 1]) - np.angle(W[x, y + 1])) if j != k else 0.0
+                out["deltas"][n, N - k - 1], out["sigmas"][n, N - k - 1] = delta, sigma
+                V = _smz(n, sigma, delta, N) @ V
+                x, y = x + 1, y + 1
+    out["zetas"] = _final_phases(V, N, rtol, atol)
+    return out
+
+
+def _absorb_zetas(ph):
+    """Move the residual phases zeta_j into the sMZ sigmas of the layers they
+    cross and into phase shifters on the last mode's edge
+    (decompositions.py:829-876)."""
+    import collections
+
+    N = ph["m"]
+    out = dict(ph)
+    zetas = out.pop("zetas")
+    out["phi_edges"] = collections.defaultdict(float)
+    if N % 2 == 0:
+        out["phi_outs"][0] = zetas[0]
+        js, shift, parity = range(1, N), 0, 1
+    else:
+        js, shift, parity = range(N), 1, 0
+    for j in js:
+        zeta = zetas[j]
+        layer = N - j - shift
+        for mode in range(j, N - 1, 2):
+            out["sigmas"][mode, layer] += zeta
+        for mode in range(j + 1, N - 1, 2):
+            out["sigmas"][mode, layer - 1] -= zeta
+        if layer % 2 == parity:
+            out["phi_edges"][N - 1, layer] += zeta
+        else:
+            out["phi_edges"][N - 1, layer - 1] -= zeta
+    return out
+
+
+def rectangular_compact(U, rtol=1e-12, atol=1e-12):
+    """Phases of the compact rectangular mesh with the residual phases on the
+    edge (decompositions.py:879-903)."""
+    U = np.asarray(U)
+    _check_unitary(U, rtol, atol)
+    return _absorb_zetas(_rectangular_compact_init(U, rtol, atol))
+
+
+def smz_gates(modes, phi1, phi2):
+    """sMZgate(phi1, phi2) on (m, n) as the fock compiler runs it
+    (ops.py:2015-2031): BS(pi/4, pi/2), R(phi2 - pi/2) on n, R(phi1 - pi/2) on m,
+    BS(pi/4, pi/2)."""
+    m, n = modes
+    return [("BS", (m, n), (np.pi / 4, np.pi / 2)), ("R", (n,), (phi2 - np.pi / 2,)),
+            ("R", (m,), (phi1 - np.pi / 2,)), ("BS", (m, n), (np.pi / 4, np.pi / 2))]
+
+
+def _compact_gates(mesh, U, tol):
+    """Gate list of a compact mesh (ops.py:2461-2499): sMZ blocks with internal
+    phases sigma +- delta, input, edge and output phase shifters."""
+    gates = []
+    if mesh == "rectangular_compact":
+        ph = rectangular_compact(U, rtol=tol, atol=tol)
+        N = ph["m"]
+        for j in range(0, N - 1, 2):
+            gates.append(("R", (j,), (ph["phi_ins"][j],)))
+        for layer in range(N):
+            if (layer + N + 1) % 2 == 0:
+                gates.append(("R", (N - 1,), (ph["phi_edges"][N - 1, layer],)))
+            for mode in range(layer % 2, N - 1, 2):
+                s, d = ph["sigmas"][mode, layer], ph["deltas"][mode, layer]
+                gates += smz_gates((mode, mode + 1), s + d, s - d)
+        for j, phi in ph["phi_outs"].items():
+            gates.append(("R", (j,), (phi,)))
+    else:
+        ph = triangular_compact(U, rtol=tol, atol=tol)
+        N = ph["m"]
+        for j in range(N - 1):
+            gates.append(("R", (j + 1,), (ph["phi_ins"][j],)))
+            for k in range(j + 1):
+                n = j - k
+                s, d = ph["sigmas"][n, k], ph["deltas"][n, k]
+                gates += smz_gates((n, n + 1), s + d, s - d)
+        for j in range(N):
+            gates.append(("R", (j,), (ph["zetas"][j],)))
+    return gates
+
+
+MESHES = ("rectangular", "rectangular_phase_end", "rectangular_symmetric", "triangular",
+          "rectangular_compact", "triangular_compact")
+# accepted by the reference, not built here (its SU(2)/SU(3) parametrisation)
+COMPACT_MESHES = ("sun_compact",)
 
 
 # reference tolerances (strawberryfields/ops.py:48-52)
@@ -215,6 +407,9 @@ def interferometer_gates(U, drop_identity=True, tol=1e-6, mesh="rectangular"):
     if mesh not in MESHES:
         raise ValueError(f"Unknown mesh '{mesh}'. Mesh must be one of {set(MESHES + COMPACT_MESHES)}")
     U = np.asarray(U)
+    if mesh in ("rectangular_compact", "triangular_compact"):
+        # no identity shortcut for these meshes in the reference (ops.py:2663-2669)
+        return _compact_gates(mesh, U, tol)
     if np.allclose(U, np.identity(len(U)), atol=DECOMPOSITION_MERGE_TOL, rtol=0) and drop_identity:
         return []
     decomp = {"rectangular": rectangular, "rectangular_phase_end": rectangular_phase_end,

@@ -395,10 +395,37 @@ def test_other_meshes_follow_their_decomposition_identities():
             g = it.interferometer_gates(U, False, mesh="triangular")
             realised = np.diag(d) @ prod([it._t(*t) for t in tl], np.identity(N))
             assert np.abs(_mode_unitary(g, N) - realised).max() < 1e-12
-    for mesh in it.MESHES:
+    for mesh in ("rectangular", "rectangular_phase_end", "rectangular_symmetric", "triangular"):
         assert it.interferometer_gates(np.eye(4), mesh=mesh) == []
     with pytest.raises(NotImplementedError):
         it.interferometer_gates(cases[0], mesh="sun_compact")
+
+
+def test_compact_meshes_reconstruct_the_unitary():
+    """The compact rectangular and triangular meshes of symmetric MZ blocks
+    (decompositions.py:636-905, commands ops.py:2461-2499): their gate lists
+    (sMZ as BS(pi/4, pi/2), R, R, BS(pi/4, pi/2)) multiply back to U for Haar
+    unitaries of size 2..8, permutations and diagonal phases, as the reference's
+    own test of these meshes asserts (tests/frontend/test_ops_decompositions.py:
+    218-262); the triangular mesh has N (N - 1) / 2 blocks; the block matrix is
+    the sMZ's, e^{i sigma} [[sin d, cos d], [cos d, -sin d]] with phases sigma +- d."""
+    from scipy.stats import unitary_group
+    from strawberryfields_amd import interferometer as it
+
+    s, d = 0.4, -0.9
+    g = it.smz_gates((0, 1), s + d, s - d)
+    assert np.abs(_mode_unitary(g, 2) - it._smz(0, s, d, 2)).max() < 1e-14
+    cases = [unitary_group.rvs(N, random_state=50 + N) for N in (2, 3, 4, 5, 6, 7, 8)]
+    cases += [np.eye(4)[[2, 0, 3, 1]], np.diag(np.exp(1j * np.array([0.3, -1.2, 2.0]))), np.identity(5)]
+    for U in cases:
+        N = len(U)
+        for mesh in ("rectangular_compact", "triangular_compact"):
+            g = it.interferometer_gates(U, mesh=mesh)
+            assert np.abs(_mode_unitary(g, N) - U).max() < 1e-11, (mesh, N)
+        ph = it.triangular_compact(U)
+        assert len(ph["deltas"]) == N * (N - 1) // 2
+    with pytest.raises(ValueError):
+        it.rectangular_compact(np.ones((3, 3)))
     with pytest.raises(ValueError):
         it.interferometer_gates(cases[0], mesh="hexagonal")
     with pytest.raises(ValueError):
@@ -426,7 +453,7 @@ def test_interferometer_frontend_issues_backend_calls():
     frontend.Engine(be, backend_options={"cutoff_dim": 3}).run(prog)
     assert {c[0] for c in be.calls[1:]} <= {"mzgate", "rotation"} and be.calls[1][0] == "mzgate"
     with pytest.raises(NotImplementedError):
-        frontend.Interferometer(U, mesh="rectangular_compact")
+        frontend.Interferometer(U, mesh="sun_compact")
     with pytest.raises(ValueError):
         frontend.Interferometer(U, mesh="hexagonal")
 
@@ -511,3 +538,19 @@ def test_preparation_ops_issue_their_backend_calls():
     want = ((Dop(al) @ vac) - (Dop(-al) @ vac))[:12]
     want /= np.sqrt(2 * (1 - np.exp(-2 * abs(al) ** 2)))
     assert np.abs(ket - want).max() < 1e-13
+
+
+def test_smz_gate_decomposes_as_the_fock_compiler_does():
+    """sMZgate(phi_in, phi_ex) reaches the backend as BS(pi/4, pi/2),
+    R(phi_ex - pi/2) on the second mode, R(phi_in - pi/2) on the first,
+    BS(pi/4, pi/2) (ops.py:2015-2031)."""
+    prog = frontend.Program(3)
+    with prog.context as q:
+        frontend.sMZgate(0.7, -0.2) | (q[2], q[0])
+    be = RecordingBackend()
+    frontend.Engine(be, backend_options={"cutoff_dim": 3}).run(prog)
+    calls = be.calls[1:]
+    assert [c[0] for c in calls] == ["beamsplitter", "rotation", "rotation", "beamsplitter"]
+    assert calls[0][1:] == (np.pi / 4, np.pi / 2, 2, 0) and calls[3][1:] == (np.pi / 4, np.pi / 2, 2, 0)
+    assert calls[1][-1] == 0 and abs(calls[1][1] - (-0.2 - np.pi / 2)) < 1e-15
+    assert calls[2][-1] == 2 and abs(calls[2][1] - (0.7 - np.pi / 2)) < 1e-15
