@@ -406,10 +406,16 @@ __global__ void __launch_bounds__(NT) k_stage_adj(const C<T>* __restrict__ psi_i
     double s = 0.0;
     for (int w = 0; w < kWaves; ++w) s += gsl[idx * kWaves + w];
     // program partial layout [((op * 2 + pass) * 2 + slot) * nwg + workgroup]:
-    // pass 0 gets the sum, pass 1 is cleared (the launch has exactly nwg groups)
-    double* base = partial + ((int64_t)a.g[gi].op * 4 + slot) * nwg + blockIdx.x;
-    base[0] = s;
-    base[2 * (int64_t)nwg] = 0.0;
+    // pass 0 gets the sum, pass 1 is cleared.  The launch may have fewer groups
+    // than nwg (one per tile at most), so group b also clears the columns
+    // b + gridDim.x, b + 2 gridDim.x, ... below nwg.
+    double* col = partial + ((int64_t)a.g[gi].op * 4 + slot) * nwg;
+    col[blockIdx.x] = s;
+    col[2 * (int64_t)nwg + blockIdx.x] = 0.0;
+    for (int w = blockIdx.x + gridDim.x; w < nwg; w += gridDim.x) {
+      col[w] = 0.0;
+      col[2 * (int64_t)nwg + w] = 0.0;
+    }
   }
 }
 

@@ -801,7 +801,10 @@ extern "C" int sfamd_prog_adjoint(const sfamd_prog_t* p, int begin, int end, con
     void* dst = (i == begin) ? g_in : ((lam == LA) ? LB : LA);
     const Step& st = p->steps[i];
     if (st.fused) {
-      RET_IF(stage_adj(p->D, st.K, saved[i - begin], lam, dst, table, st.args, nwg, partial, nwg, p->dtype, s));
+      // one workgroup per tile at most; the kernel clears the partial columns
+      // of the groups it does not launch
+      const int grid = (int)(st.args.ntiles < nwg ? st.args.ntiles : nwg);
+      RET_IF(stage_adj(p->D, st.K, saved[i - begin], lam, dst, table, st.args, grid, partial, nwg, p->dtype, s));
     } else if (st.diag) {
       RET_IF(diag_run_adj(saved[i - begin], lam, dst, table, st.run, partial, nwg, p->dtype, s));
     } else {
