@@ -90,9 +90,14 @@ __host__ __device__ __forceinline__ int64_t ipow(int64_t b, int e) {
 __host__ __device__ __forceinline__ int bs_lo(int s, int D) { return s - D + 1 > 0 ? s - D + 1 : 0; }
 __host__ __device__ __forceinline__ int bs_hi(int s, int D) { return s < D - 1 ? s : D - 1; }
 __host__ __device__ __forceinline__ int64_t sum_sq(int64_t x) { return x * (x + 1) * (2 * x + 1) / 6; }
+// block offsets in 32-bit unsigned arithmetic: for s <= 2D - 1 <= 79 the
+// products stay below 2^32, and the division by 6 becomes a multiply-high on
+// the GPU instead of a 64-bit division sequence (it runs per amplitude in the
+// element-wise and stage kernels)
+__host__ __device__ __forceinline__ unsigned sum_sq_u32(unsigned x) { return x * (x + 1) * (2 * x + 1) / 6u; }
 __host__ __device__ __forceinline__ int64_t bs_block_off(int s, int D) {
-  if (s <= D) return sum_sq(s);
-  return sum_sq(D) + sum_sq(D - 1) - sum_sq(2 * D - s - 1);
+  if (s <= D) return (int64_t)sum_sq_u32((unsigned)s);
+  return (int64_t)(sum_sq_u32((unsigned)D) + sum_sq_u32((unsigned)(D - 1)) - sum_sq_u32((unsigned)(2 * D - s - 1)));
 }
 __host__ __device__ __forceinline__ int64_t bs_nb(int D) { return bs_block_off(2 * D - 1, D); }
 
