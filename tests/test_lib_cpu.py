@@ -554,3 +554,14 @@ def test_smz_gate_decomposes_as_the_fock_compiler_does():
     assert calls[0][1:] == (np.pi / 4, np.pi / 2, 2, 0) and calls[3][1:] == (np.pi / 4, np.pi / 2, 2, 0)
     assert calls[1][-1] == 0 and abs(calls[1][1] - (-0.2 - np.pi / 2)) < 1e-15
     assert calls[2][-1] == 2 and abs(calls[2][1] - (0.7 - np.pi / 2)) < 1e-15
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU; --gpus N > 1 exits with a message and a
+    non-zero status before touching a device."""
+    import subprocess
+    import sys
+
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode != 0 and "one GPU" in (r.stdout + r.stderr)
