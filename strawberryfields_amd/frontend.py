@@ -410,20 +410,18 @@ class GKP(Operation):
 class Interferometer(Operation):
     """Passive linear interferometer U (ops.py:2556-2718), compiled as the fock
     compiler does: Rgate and BSgate for the rectangular, rectangular_phase_end,
-    triangular and the two compact meshes (their sMZ blocks decomposed), MZgate
-    and Rgate for rectangular_symmetric.  sun_compact raises
-    NotImplementedError; an unknown mesh raises ValueError as in ops.py:2638-2651."""
+    triangular and the compact meshes (their sMZ and SU(2) blocks decomposed),
+    MZgate and Rgate for rectangular_symmetric; an unknown mesh raises ValueError
+    as in ops.py:2638-2651."""
 
     def __init__(self, U, mesh="rectangular", drop_identity=True, tol=1e-6):
-        from .interferometer import COMPACT_MESHES, MESHES
+        from .interferometer import MESHES
 
         U = np.asarray(U)
         super().__init__([U])
         self.ns = U.shape[0]
-        if mesh not in MESHES + COMPACT_MESHES:
-            raise ValueError(f"Unknown mesh '{mesh}'. Mesh must be one of {set(MESHES + COMPACT_MESHES)}")
-        if mesh in COMPACT_MESHES:
-            raise NotImplementedError(f"mesh {mesh!r} is not implemented; use one of {MESHES}")
+        if mesh not in MESHES:
+            raise ValueError(f"Unknown mesh '{mesh}'. Mesh must be one of {set(MESHES)}")
         self.mesh, self.drop_identity, self.tol = mesh, drop_identity, tol
 
     def decompose(self, reg):

@@ -380,10 +380,156 @@ def _compact_gates(mesh, U, tol):
     return gates
 
 
+# ---- SU(n) factorisation into SU(2) blocks on neighbouring modes
+# (decompositions.py:1073-1381): U = e^{i phi / n} S, S in SU(n) reduced by a
+# staircase of SU(2) rotations to SU(n - 1) down to SU(3), which factors as
+# SU_23 SU_12 SU_23 (Rowe et al., J. Math. Phys. 40, 3604 (1999)).  A block
+# with Euler angles (a, b, g) is diag(e^{ia/2}, e^{-ia/2}) [[cos b/2, -sin b/2],
+# [sin b/2, cos b/2]] diag(e^{ig/2}, e^{-ig/2}).
+
+
+def _su2_params(U, tol=1e-10):
+    """Euler angles [a, b, g] of a 2x2 special unitary (decompositions.py:1293-1335)."""
+    if U.shape != (2, 2):
+        raise ValueError("Input matrix dimensions of _su2_parameters must be 2x2.")
+    if not np.isclose(np.linalg.det(U), 1, atol=tol, rtol=0):
+        raise ValueError("Input matrix must have determinant 1 to be decomposed into SU(2) parameters.")
+    if np.isclose(np.abs(U[0, 1]), 1, atol=tol, rtol=0):
+        b = 2 * np.arcsin(1)
+    else:
+        b = 2 * np.arcsin(np.abs(U[0, 1]))
+    half_sum = np.angle(U[0, 0])  # (a + g) / 2
+    half_diff = -np.angle(U[1, 0])  # (a - g) / 2
+    return [half_sum + half_diff, b, half_sum - half_diff]
+
+
+def _su3_params(U):
+    """Three SU(2) blocks (on modes 2-3, 1-2, 2-3) of a 3x3 special unitary,
+    with the identity-column and phase-only special cases
+    (decompositions.py:1338-1381)."""
+    if U.shape != (3, 3):
+        raise ValueError("Input matrix dimensions of _su3_parameters must be 3x3.")
+    if not np.isclose(np.linalg.det(U), 1):
+        raise ValueError("Input matrix must have determinant 1 to be decomposed into SU(2) parameters.")
+    x, y, z = U[0, 0], U[1, 0], U[2, 0]
+    if np.isclose(x, 1):
+        return [[0.0, 0.0, 0.0], [0.0, 0.0, 0.0], _su2_params(U[1:, 1:])]
+    if np.isclose(np.abs(x), 1):
+        ph = np.array([[np.conj(x), 0], [0, x]])
+        full = np.identity(3, dtype=complex)
+        full[0:2, 0:2] = ph
+        rest = (full @ U)[1:, 1:]
+        return [[0.0, 0.0, 0.0], _su2_params(ph.conj().T), _su2_params(rest)]
+    cf = np.sqrt(1 - np.abs(x) ** 2)
+    Y, Z = y / cf, z / cf
+    left = np.array([[1, 0, 0], [0, Y, -np.conj(Z)], [0, Z, np.conj(Y)]])
+    middle = np.array([[x, -cf, 0], [cf, np.conj(x), 0], [0, 0, 1]])
+    right = middle.conj().T @ left.conj().T @ U
+    return [_su2_params(left[1:, 1:]), _su2_params(middle[0:2, 0:2]), _su2_params(right[1:, 1:])]
+
+
+def _staircase(U, rtol, atol):
+    """SU(2) blocks on modes (n-1, n), ..., (1, 2) that reduce the first column
+    of U to e_1, and the reduced matrix (decompositions.py:1187-1290), with the
+    reference's branches for a column that already has a single nonzero entry."""
+    n = U.shape[0]
+    out = []
+    cur = U
+    moduli = np.abs(U[:, 0])
+    if np.allclose(sorted(moduli), [0.0] * (n - 1) + [1], rtol, atol):
+        if np.isclose(cur[0, 0], 1, rtol, atol):
+            out = [[0.0, 0.0, 0.0]] * (n - 1)
+        elif np.isclose(np.abs(cur[0, 0]), 1, rtol, atol):
+            ph = np.array([[cur[0, 0].conjugate(), 0], [0, cur[0, 0]]])
+            out = [[0.0, 0.0, 0.0]] * (n - 2) + [_su2_params(ph.conj().T)]
+            full = np.identity(n, dtype=complex)
+            full[0:2, 0:2] = ph
+            cur = full @ cur
+        else:
+            for r in range(n - 1, 0, -1):
+                if not np.isclose(cur[r, 0], 0, rtol, atol):
+                    perm = np.array([[0, -1], [1, 0]])
+                    full = np.identity(n, dtype=complex)
+                    full[r - 1 : r + 1, r - 1 : r + 1] = perm
+                    tmp = full @ cur
+                    if r == 1:  # at the top the phase of the entry joins the block
+                        perm = np.array([[tmp[0, 0].conjugate(), 0], [0, tmp[0, 0]]]) @ perm
+                    out.append(_su2_params(perm.conj().T))
+                    full = np.identity(n, dtype=complex)
+                    full[r - 1 : r + 1, r - 1 : r + 1] = perm
+                    cur = full @ cur
+                else:
+                    out.append([0, 0, 0])
+    else:
+        for t in range(n - 1):
+            i, j = n - t - 2, n - t - 1
+            full = np.identity(n, dtype=complex)
+            if t != n - 2:
+                cf = np.sqrt(1 - sum(np.abs(cur[k, 0]) ** 2 for k in range(i)))
+                Y, Z = cur[i, 0] / cf, cur[j, 0] / cf
+                inv = np.array([[np.conj(Y), np.conj(Z)], [-Z, Y]])
+            else:
+                x = U[0, 0]
+                cf = np.sqrt(1 - np.abs(x) ** 2)
+                inv = np.array([[np.conj(x), cf], [-cf, x]])
+            out.append(_su2_params(inv.conj().T))
+            full[i : j + 1, i : j + 1] = inv
+            cur = full @ cur
+    return out, cur
+
+
+def _sun_params(U, rtol, atol):
+    if U.shape == (3, 3):
+        return _su3_params(U)
+    stairs, rest = _staircase(U, rtol, atol)
+    return stairs + _sun_params(rest[1:, 1:], rtol, atol)
+
+
+def sun_compact(U, rtol=1e-12, atol=1e-12):
+    """([((i, i + 1), [a, b, g]), ...], global phase or None) of the SU(n)
+    factorisation (decompositions.py:1073-1155); n >= 3.  U is taken as complex,
+    so a real orthogonal U with det -1 gets the complex root det^(-1/n) (the same
+    values the reference computes for a complex-typed U) rather than a NaN."""
+    U = np.asarray(U, dtype=np.complex128)
+    n = U.shape[0]
+    if n < 3:
+        raise ValueError("Input matrix for decomposition must be at least 3x3.")
+    if not np.allclose(U @ U.conj().T, np.identity(n), rtol=rtol, atol=atol):
+        raise ValueError("The input matrix is not unitary.")
+    det = np.linalg.det(U)
+    SU = np.array(U, dtype=complex)
+    phase = None
+    if not np.isclose(det, 1, rtol=rtol, atol=atol):
+        SU = SU * det ** (-1 / n)
+        phase = float(np.angle(det))
+    flat = _sun_params(SU, rtol, atol)
+    out, k = [], 0
+    for m2 in range(2, n + 1):
+        for m1 in range(n - 1, m2 - 2, -1):
+            out.append([(m1 - 1, m1), flat[k]])
+            k += 1
+    return out, phase
+
+
+def _sun_gates(U, tol):
+    """Gate list of the sun_compact mesh (ops.py:2502-2526): per block, in matrix
+    order, R(a/2) and R(-a/2), BS(b/2, 0), R(g/2) and R(-g/2), after the global
+    phase R(phi / n) on every mode; the list is then reversed into application
+    order."""
+    params, phase = sun_compact(U, rtol=tol, atol=tol)
+    n = np.asarray(U).shape[0]
+    cmds = []
+    if phase is not None:
+        cmds += [("R", (m,), (phase / n,)) for m in range(n)]
+    for (m1, m2), (a, b, g) in params:
+        cmds += [("R", (m1,), (a / 2,)), ("R", (m2,), (-a / 2,)), ("BS", (m1, m2), (b / 2, 0.0)),
+                 ("R", (m1,), (g / 2,)), ("R", (m2,), (-g / 2,))]
+    return cmds[::-1]
+
+
+# every mesh the reference's Interferometer accepts (ops.py:2638-2646)
 MESHES = ("rectangular", "rectangular_phase_end", "rectangular_symmetric", "triangular",
-          "rectangular_compact", "triangular_compact")
-# accepted by the reference, not built here (its SU(2)/SU(3) parametrisation)
-COMPACT_MESHES = ("sun_compact",)
+          "rectangular_compact", "triangular_compact", "sun_compact")
 
 
 # reference tolerances (strawberryfields/ops.py:48-52)
@@ -396,20 +542,21 @@ def interferometer_gates(U, drop_identity=True, tol=1e-6, mesh="rectangular"):
     (ops.py:2655-2718): per first-list block R(phi) and BS(theta, 0), or
     MZgate(phi_in, phi_ex) for the symmetric mesh; the diagonal phases; then, for
     the rectangular mesh, BS(-theta, 0) and R(-phi) per left block in reverse.
+    The compact meshes compile to R and BS through their sMZ / SU(2) blocks.
 
     The triangular mesh is compiled with the same per-block rule as the
     reference compiles it (R(phi), BS(theta, 0)), which realises T where the
     decomposition's identity needs T^{-1}: the reference's own decomposition
     test leaves it out (tests/frontend/test_ops_decompositions.py:32-41), and
     the gate list here is the reference's, not a corrected one."""
-    if mesh in COMPACT_MESHES:
-        raise NotImplementedError(f"mesh {mesh!r} is not implemented; use one of {MESHES}")
     if mesh not in MESHES:
-        raise ValueError(f"Unknown mesh '{mesh}'. Mesh must be one of {set(MESHES + COMPACT_MESHES)}")
+        raise ValueError(f"Unknown mesh '{mesh}'. Mesh must be one of {set(MESHES)}")
     U = np.asarray(U)
     if mesh in ("rectangular_compact", "triangular_compact"):
-        # no identity shortcut for these meshes in the reference (ops.py:2663-2669)
+        # no identity shortcut for these meshes in the reference (ops.py:2663-2673)
         return _compact_gates(mesh, U, tol)
+    if mesh == "sun_compact":
+        return _sun_gates(U, tol)
     if np.allclose(U, np.identity(len(U)), atol=DECOMPOSITION_MERGE_TOL, rtol=0) and drop_identity:
         return []
     decomp = {"rectangular": rectangular, "rectangular_phase_end": rectangular_phase_end,

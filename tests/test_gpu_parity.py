@@ -1073,7 +1073,7 @@ def test_engine_collects_measurement_samples():
 @pytest.mark.parametrize("dtype", [C128, C64])
 @pytest.mark.parametrize("pure", [True, False])
 @pytest.mark.parametrize("mesh", ["rectangular", "rectangular_phase_end", "rectangular_symmetric", "rectangular_compact",
-                                  "triangular_compact"])
+                                  "triangular_compact", "sun_compact"])
 def test_interferometer_matches_exact_passive_action(dtype, pure, mesh):
     """Interferometer(U) (rectangular meshes on the BS / R kernels, the
     symmetric mesh on the MZ kernels) on a 4-mode state with fewer than D

@@ -397,13 +397,16 @@ def test_other_meshes_follow_their_decomposition_identities():
             assert np.abs(_mode_unitary(g, N) - realised).max() < 1e-12
     for mesh in ("rectangular", "rectangular_phase_end", "rectangular_symmetric", "triangular"):
         assert it.interferometer_gates(np.eye(4), mesh=mesh) == []
-    with pytest.raises(NotImplementedError):
-        it.interferometer_gates(cases[0], mesh="sun_compact")
+    with pytest.raises(ValueError):
+        it.interferometer_gates(cases[0], mesh="sun_compact")  # needs n >= 3
 
 
 def test_compact_meshes_reconstruct_the_unitary():
     """The compact rectangular and triangular meshes of symmetric MZ blocks
-    (decompositions.py:636-905, commands ops.py:2461-2499): their gate lists
+    (decompositions.py:636-905, commands ops.py:2461-2499) and the SU(n)
+    factorisation of sun_compact (decompositions.py:1073-1381, commands
+    ops.py:2502-2526; n >= 3, with its special branches for columns holding a
+    single nonzero entry, hit by the permutations): their gate lists
     (sMZ as BS(pi/4, pi/2), R, R, BS(pi/4, pi/2)) multiply back to U for Haar
     unitaries of size 2..8, permutations and diagonal phases, as the reference's
     own test of these meshes asserts (tests/frontend/test_ops_decompositions.py:
@@ -416,10 +419,11 @@ def test_compact_meshes_reconstruct_the_unitary():
     g = it.smz_gates((0, 1), s + d, s - d)
     assert np.abs(_mode_unitary(g, 2) - it._smz(0, s, d, 2)).max() < 1e-14
     cases = [unitary_group.rvs(N, random_state=50 + N) for N in (2, 3, 4, 5, 6, 7, 8)]
-    cases += [np.eye(4)[[2, 0, 3, 1]], np.diag(np.exp(1j * np.array([0.3, -1.2, 2.0]))), np.identity(5)]
+    cases += [np.eye(4)[[2, 0, 3, 1]], np.diag(np.exp(1j * np.array([0.3, -1.2, 2.0]))), np.identity(5),
+              np.eye(5)[::-1], np.eye(3)[[1, 0, 2]] * 1j]
     for U in cases:
         N = len(U)
-        for mesh in ("rectangular_compact", "triangular_compact"):
+        for mesh in ("rectangular_compact", "triangular_compact") + (("sun_compact",) if N >= 3 else ()):
             g = it.interferometer_gates(U, mesh=mesh)
             assert np.abs(_mode_unitary(g, N) - U).max() < 1e-11, (mesh, N)
         ph = it.triangular_compact(U)
@@ -452,8 +456,6 @@ def test_interferometer_frontend_issues_backend_calls():
     be = RecordingBackend()
     frontend.Engine(be, backend_options={"cutoff_dim": 3}).run(prog)
     assert {c[0] for c in be.calls[1:]} <= {"mzgate", "rotation"} and be.calls[1][0] == "mzgate"
-    with pytest.raises(NotImplementedError):
-        frontend.Interferometer(U, mesh="sun_compact")
     with pytest.raises(ValueError):
         frontend.Interferometer(U, mesh="hexagonal")
 
