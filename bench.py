@@ -487,8 +487,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(
-            "bench.py measures one GPU: this build has no multi-GPU path "
-            "(the sharding design is described in DESIGN.md); run with --gpus 1",
+            "bench.py measures one GPU: there is no multi-GPU path "
+            "(one state would be sharded along its most significant mode axis); run with --gpus 1",
             file=sys.stderr,
         )
         sys.exit(2)

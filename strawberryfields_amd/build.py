@@ -19,7 +19,8 @@ LIB_PATH = os.path.join(PKG_DIR, "libsfamd.so")
 
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 
-# -packed-fp32-ops: no v_pk_*_f32 in the device code (build flags)
+# -packed-fp32-ops: the device code is built without packed FP32 instructions
+# (v_pk_*_f32); the host pass warns that it does not know the feature
 HIP_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
