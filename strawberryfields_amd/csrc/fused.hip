@@ -110,14 +110,16 @@ __device__ __forceinline__ void stage_all_matrices(const C<T>* __restrict__ tabl
 }
 
 // ---- one gate on a tile: dst = G src --------------------------------------
-template <typename T, int D, int K, int NT>
-__device__ __forceinline__ void apply_gate(const sfamd::StageGate& g, const C<T>* __restrict__ mat,
-                                           const C<T>* __restrict__ src, C<T>* __restrict__ dst) {
+// `put(e, v)` stores output element e: into the next LDS tile, or straight to
+// global memory for the last gate of a forward stage.
+template <typename T, int D, int K, int NT, typename Put>
+__device__ __forceinline__ void apply_gate_to(const sfamd::StageGate& g, const C<T>* __restrict__ mat,
+                                              const C<T>* __restrict__ src, Put put) {
   constexpr int DK = Pow<D, K>::v;
   const int a0 = g.a0, a1 = g.a1;
   const int s0 = pw<D, K>(K - 1 - a0);
   if (g.kind == SFAMD_OP_ROTATION || g.kind == SFAMD_OP_KERR) {
-    for (int e = threadIdx.x; e < DK; e += NT) dst[e] = cmul(mat[digit<D, K>(e, a0)], src[e]);
+    for (int e = threadIdx.x; e < DK; e += NT) put(e, cmul(mat[digit<D, K>(e, a0)], src[e]));
   } else if (g.kind == SFAMD_OP_BEAMSPLITTER) {
     const int s1 = pw<D, K>(K - 1 - a1);
     for (int e = threadIdx.x; e < DK; e += NT) {
@@ -134,7 +136,7 @@ __device__ __forceinline__ void apply_gate(const sfamd::StageGate& g, const C<T>
       for (int k = 0; k < D; ++k) {
         if (k >= lo && k <= hi) cmac(acc, row[k - lo], src[base + k * s0 + (s - k) * s1]);
       }
-      dst[e] = acc;
+      put(e, acc);
     }
   } else if (g.kind == SFAMD_OP_SQUEEZE) {  // zero unless i + k even: half the terms
     for (int e = threadIdx.x; e < DK; e += NT) {
@@ -148,7 +150,7 @@ __device__ __forceinline__ void apply_gate(const sfamd::StageGate& g, const C<T>
         const int k = 2 * h + p;
         if (k < D) cmac(acc, row[k], src[base + k * s0]);
       }
-      dst[e] = acc;
+      put(e, acc);
     }
   } else {  // dense single-mode
     for (int e = threadIdx.x; e < DK; e += NT) {
@@ -158,9 +160,15 @@ __device__ __forceinline__ void apply_gate(const sfamd::StageGate& g, const C<T>
       C<T> acc = mk<T>(0, 0);
 #pragma unroll
       for (int k = 0; k < D; ++k) cmac(acc, row[k], src[base + k * s0]);
-      dst[e] = acc;
+      put(e, acc);
     }
   }
+}
+
+template <typename T, int D, int K, int NT>
+__device__ __forceinline__ void apply_gate(const sfamd::StageGate& g, const C<T>* __restrict__ mat,
+                                           const C<T>* __restrict__ src, C<T>* __restrict__ dst) {
+  apply_gate_to<T, D, K, NT>(g, mat, src, [dst](int e, C<T> v) { dst[e] = v; });
 }
 
 // ---- adjoint of one gate on a tile -----------------------------------------
@@ -336,14 +344,17 @@ __global__ void __launch_bounds__(NT) k_stage_fwd(const C<T>* __restrict__ in, C
     __syncthreads();
     C<T>* src = bufA;
     C<T>* dst = bufB;
-    for (int gi = 0; gi < a.ngates; ++gi) {
+    for (int gi = 0; gi < a.ngates - 1; ++gi) {
       apply_gate<T, D, K, NT>(a.g[gi], mats + a.g[gi].moff, src, dst);
       __syncthreads();
       C<T>* tmp = src;
       src = dst;
       dst = tmp;
     }
-    for (int e = threadIdx.x; e < DK; e += NT) out[base + local_offset<D, K>(e, a)] = src[e];
+    // the last gate writes its outputs straight to the state
+    const int gl = a.ngates - 1;
+    apply_gate_to<T, D, K, NT>(a.g[gl], mats + a.g[gl].moff, src,
+                               [&](int e, C<T> v) { out[base + local_offset<D, K>(e, a)] = v; });
     __syncthreads();
   }
 }
