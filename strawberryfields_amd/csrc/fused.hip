@@ -179,10 +179,13 @@ __device__ __forceinline__ void apply_gate(const sfamd::StageGate& g, const C<T>
 // which summed over k is Re sum conj(lam_out) (dG/dp psi_in).  The phase
 // derivatives come from G itself: d/dphi of D, S, BS is i (m - n) f G with
 // f = 1, 1/2, 1 (row m = out index, column n = in index).
-template <typename T, int D, int K, int NT>
+// `put(e, v)` stores lam_in element e: into the other LDS cotangent tile, or
+// straight to global memory for the first gate of the stage (the last one the
+// backward sweep visits).
+template <typename T, int D, int K, int NT, typename Put>
 __device__ __forceinline__ void adjoint_gate(const sfamd::StageGate& g, const C<T>* __restrict__ mat,
                                              const C<T>* __restrict__ psi, const C<T>* __restrict__ lo_,
-                                             C<T>* __restrict__ li, double& g0, double& g1) {
+                                             Put put, double& g0, double& g1) {
   constexpr int DK = Pow<D, K>::v;
   const int a0 = g.a0, a1 = g.a1;
   const int s0 = pw<D, K>(K - 1 - a0);
@@ -192,7 +195,7 @@ __device__ __forceinline__ void adjoint_gate(const sfamd::StageGate& g, const C<
       const int i = digit<D, K>(e, a0);
       const C<T> f = mat[i];
       const C<T> l = lo_[e];
-      li[e] = cmul(cconj(f), l);
+      put(e, cmul(cconj(f), l));
       const T w = (T)(kerr ? i * i : i);
       const C<T> fx = cmul(f, psi[e]);
       g0 += (double)re_conj_mul(l, mk<T>(-w * fx.im, w * fx.re));
@@ -223,7 +226,7 @@ __device__ __forceinline__ void adjoint_gate(const sfamd::StageGate& g, const C<
           w1.im += w * lb.re;
         }
       }
-      li[e] = acc;
+      put(e, acc);
       const C<T> x = psi[e];
       g0 += (double)(w0.re * x.re - w0.im * x.im);
       g1 += (double)(w1.re * x.re - w1.im * x.im);
@@ -249,7 +252,7 @@ __device__ __forceinline__ void adjoint_gate(const sfamd::StageGate& g, const C<
           w1.im += w * lu.re;
         }
       }
-      li[e] = acc;
+      put(e, acc);
       const C<T> x = psi[e];
       g0 += (double)(w0.re * x.re - w0.im * x.im);
       g1 += (double)(w1.re * x.re - w1.im * x.im);
@@ -394,7 +397,12 @@ __global__ void __launch_bounds__(NT) k_stage_adj(const C<T>* __restrict__ psi_i
     C<T>* li = lamB;
     for (int gi = a.ngates - 1; gi >= 0; --gi) {
       double g0 = 0.0, g1 = 0.0;
-      adjoint_gate<T, D, K, NT>(a.g[gi], mats + a.g[gi].amoff, tiles + gi * DK, lo_, li, g0, g1);
+      if (gi > 0)
+        adjoint_gate<T, D, K, NT>(a.g[gi], mats + a.g[gi].amoff, tiles + gi * DK, lo_,
+                                  [li](int e, C<T> v) { li[e] = v; }, g0, g1);
+      else  // the stage's first gate: lam_in goes straight to the output state
+        adjoint_gate<T, D, K, NT>(a.g[gi], mats + a.g[gi].amoff, tiles + gi * DK, lo_,
+                                  [&](int e, C<T> v) { lam_in[base + local_offset<D, K>(e, a)] = v; }, g0, g1);
       // R and K have one parameter: their second slot stays at its cleared zero
       const bool two = a.g[gi].kind != SFAMD_OP_ROTATION && a.g[gi].kind != SFAMD_OP_KERR;
       g0 = wave_sum(g0);
@@ -407,9 +415,7 @@ __global__ void __launch_bounds__(NT) k_stage_adj(const C<T>* __restrict__ psi_i
       C<T>* tmp = lo_;
       lo_ = li;
       li = tmp;
-    }
-    for (int e = threadIdx.x; e < DK; e += NT) lam_in[base + local_offset<D, K>(e, a)] = lo_[e];
-    __syncthreads();
+    }  // the sweep's last barrier also frees the tiles for the next tile's loads
   }
   __syncthreads();
   for (int idx = threadIdx.x; idx < 2 * a.ngates; idx += NT) {
