@@ -19,7 +19,8 @@ What it restates (reference files, read as text, never imported):
 * state updates -- ``strawberryfields/backends/fockbackend/circuit.py``:
   ``apply_gate_BLAS`` 118-217 (the slice loop with ``np.dot``), the
   selection-rule beamsplitter loop ``_apply_two_mode_passive`` 306-335, the
-  difference-conserving ``_apply_S2`` 337-365,
+  difference-conserving ``_apply_S2`` 337-365 (both numba in the reference;
+  in C here, ``oracle/fock_cpu.c``, for the faithful mode),
   ``_apply_channel`` 65-87, ``prepare_multimode`` 393-473, ``norm`` 367-371,
   ``is_vacuum`` 600-609; ``ops.mix`` 110-120, ``partial_trace`` 144-157.
 * readout -- ``strawberryfields/backends/states.py`` ``BaseFockState``:
@@ -45,12 +46,45 @@ known-answer tests (TMSV amplitudes, MZ on |1,1>, cross-Kerr and cubic phase on
 uniform states, coherent and squeezed states, Hong-Ou-Mandel, loss).
 """
 from itertools import product
+import ctypes
+import os
 import string
 
 import numpy as np
 
 DEF_TYPE = np.complex128
 _IDX = string.ascii_lowercase
+
+# The faithful mode's selection-rule loops (numba-compiled native code in the
+# reference) run from oracle/libfock_cpu.so (oracle/fock_cpu.c, gcc -O3) when
+# it is built, else as Python loops with the same arithmetic.
+_NATIVE = {}
+
+
+def native_lib():
+    """The oracle's C loops (ctypes) or None when oracle/libfock_cpu.so is absent."""
+    if "lib" not in _NATIVE:
+        path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libfock_cpu.so")
+        lib = None
+        if os.path.exists(path):
+            lib = ctypes.CDLL(path)
+            for fn in (lib.fock_two_mode_passive, lib.fock_two_mode_diffcons):
+                fn.restype = None
+                fn.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_long]
+        _NATIVE["lib"] = lib
+    return _NATIVE["lib"]
+
+
+def _native_rule(name, mat, state, D):
+    lib = native_lib()
+    if lib is None:
+        return None
+    m = np.ascontiguousarray(mat, dtype=DEF_TYPE)
+    st = np.ascontiguousarray(state, dtype=DEF_TYPE)
+    ret = np.empty_like(st)
+    rest = st.size // (D * D)
+    getattr(lib, name)(m.ctypes.data, st.ctypes.data, ret.ctypes.data, D, rest)
+    return ret
 
 
 # ----------------------------------------------------------------- matrices
@@ -390,6 +424,10 @@ class Circuit:
     @staticmethod
     def _two_mode_passive(mat, state, D, faithful):
         """ret[i, j, ...] = sum_k mat[i, k, j, i+j-k] state[k, i+j-k, ...]  (circuit.py:306-335)."""
+        if faithful:
+            ret = _native_rule("fock_two_mode_passive", mat, state, D)
+            if ret is not None:
+                return ret
         ret = np.zeros_like(state, dtype=DEF_TYPE)
         if faithful:
             for i in range(D):
@@ -409,6 +447,10 @@ class Circuit:
     @staticmethod
     def _two_mode_diffcons(mat, state, D, faithful):
         """ret[i, k, ...] = sum_j mat[i, j, k, k+j-i] state[j, k+j-i, ...]  (circuit.py:337-365)."""
+        if faithful:
+            ret = _native_rule("fock_two_mode_diffcons", mat, state, D)
+            if ret is not None:
+                return ret
         ret = np.zeros_like(state, dtype=DEF_TYPE)
         if faithful:
             for i in range(D):

@@ -88,6 +88,12 @@ class HIPFockBackend(BaseFock):
             raise ValueError("Argument 'pure' must be either True or False")
         if cutoff_dim < 1 or cutoff_dim > _lib.MAX_CUTOFF:
             raise ValueError("Argument 'cutoff_dim' must be in [1, %d]" % _lib.MAX_CUTOFF)
+        if kwargs.get("batch_size", None) is not None:
+            # the TF backend's parameter-batch axis (tfbackend/backend.py:84-106,
+            # tfbackend/circuit.py:129-173) has no counterpart here: batches of
+            # circuits run through runtime.BatchEvaluator instead
+            raise NotImplementedError("The hipfock backend does not support batch_size; "
+                                      "use strawberryfields_amd.runtime.BatchEvaluator for many circuits")
         dtype = kwargs.get("dtype", torch.complex128)
         if dtype not in _DTYPES:
             raise ValueError("Argument 'dtype' must be complex64 or complex128")

@@ -391,9 +391,9 @@ def _compact_gates(mesh, U, tol):
 def _su2_params(U, tol=1e-10):
     """Euler angles [a, b, g] of a 2x2 special unitary (decompositions.py:1293-1335)."""
     if U.shape != (2, 2):
-        raise ValueError("Input matrix dimensions of _su2_parameters must be 2x2.")
+        raise ValueError("an SU(2) block needs a 2x2 matrix, got shape %s" % (U.shape,))
     if not np.isclose(np.linalg.det(U), 1, atol=tol, rtol=0):
-        raise ValueError("Input matrix must have determinant 1 to be decomposed into SU(2) parameters.")
+        raise ValueError("an SU(2) block needs determinant 1, got %s" % np.linalg.det(U))
     if np.isclose(np.abs(U[0, 1]), 1, atol=tol, rtol=0):
         b = 2 * np.arcsin(1)
     else:
@@ -408,9 +408,9 @@ def _su3_params(U):
     with the identity-column and phase-only special cases
     (decompositions.py:1338-1381)."""
     if U.shape != (3, 3):
-        raise ValueError("Input matrix dimensions of _su3_parameters must be 3x3.")
+        raise ValueError("the SU(3) base needs a 3x3 matrix, got shape %s" % (U.shape,))
     if not np.isclose(np.linalg.det(U), 1):
-        raise ValueError("Input matrix must have determinant 1 to be decomposed into SU(2) parameters.")
+        raise ValueError("the SU(3) base needs determinant 1, got %s" % np.linalg.det(U))
     x, y, z = U[0, 0], U[1, 0], U[2, 0]
     if np.isclose(x, 1):
         return [[0.0, 0.0, 0.0], [0.0, 0.0, 0.0], _su2_params(U[1:, 1:])]
@@ -493,7 +493,7 @@ def sun_compact(U, rtol=1e-12, atol=1e-12):
     U = np.asarray(U, dtype=np.complex128)
     n = U.shape[0]
     if n < 3:
-        raise ValueError("Input matrix for decomposition must be at least 3x3.")
+        raise ValueError("sun_compact needs a matrix of size 3 or more, got %d" % n)
     if not np.allclose(U @ U.conj().T, np.identity(n), rtol=rtol, atol=atol):
         raise ValueError("The input matrix is not unitary.")
     det = np.linalg.det(U)

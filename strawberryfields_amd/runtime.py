@@ -687,6 +687,9 @@ class GradEvaluator:
                 _lib.call("sfamd_graph_launch", self.graph, ctypes.c_void_p(self.stream.cuda_stream))
             else:
                 self._enqueue()
+        # the replay rewrote the shared gate table: a pending autograd backward
+        # of the same plan must rebuild it from its own parameters
+        self.cc._built = None
         cur.wait_stream(self.stream)
         return self.out[2], self.grads.reshape(self.cc.nops, 2)
 
@@ -906,6 +909,8 @@ class BatchEvaluator:
                 _lib.call("sfamd_graph_launch", self.graph, ctypes.c_void_p(self.main.cuda_stream))
             else:
                 self._enqueue_all()
+        for it in self.items:  # gate tables rewritten (see GradEvaluator.__call__)
+            it["cc"]._built = None
         cur.wait_stream(self.main)
         return self.losses, [it["grads"].reshape(-1, 2) for it in self.items]
 

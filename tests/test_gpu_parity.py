@@ -1,7 +1,7 @@
 """GPU parity: the HIP path against the CPU oracle on the same seeded inputs.
 
 Tolerances: complex128 max|d psi| <= 1e-12 (1e-11 for long circuits),
-complex64 <= 2e-5; gradients against central finite differences of the
+complex64 <= 1e-5; gradients against central finite differences of the
 complex128 oracle.
 """
 import itertools
@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 C128, C64 = torch.complex128, torch.complex64
-TOL = {C128: 1e-12, C64: 2e-5}
+TOL = {C128: 1e-12, C64: 1e-5}
 DEV = "cuda:0"
 
 
@@ -233,7 +233,7 @@ def test_config2_forward(dtype):
     be = run_backend(c2["ops"], 4, 15, dtype)
     st = be.state()
     ket = to_np(st.ket())
-    tol = 1e-11 if dtype == C128 else 2e-5
+    tol = 1e-12 if dtype == C128 else 1e-5
     assert np.abs(ket - ref).max() < tol
     assert abs(st.trace() - fo.trace(ref, 4, True)) < tol
     for idx in [(0, 0, 0, 0), (1, 0, 2, 1), (3, 3, 0, 1)]:
@@ -245,7 +245,7 @@ def test_config3_forward_complex64():
     ref, _ = fo.run_ops(c3["ops"], 6, 8)
     st = run_backend(c3["ops"], 6, 8, C64).state()
     ket = to_np(st.ket())
-    assert np.abs(ket - ref).max() < 2e-5
+    assert np.abs(ket - ref).max() < 1e-5
     assert 1 - _fidelity(ket, ref) < 1e-6
 
 
@@ -256,7 +256,7 @@ def test_mixed_layer_with_loss_matches_oracle():
     for dtype in (C128, C64):
         be = run_backend(ops, 3, 6, dtype, pure=False)
         ref, _ = fo.run_ops(ops, 3, 6, pure=False)
-        assert np.abs(to_np(be.state().dm()) - ref).max() < TOL[dtype] * 10
+        assert np.abs(to_np(be.state().dm()) - ref).max() < TOL[dtype]
 
 
 # ------------------------------------------------------------------ readout
@@ -426,7 +426,7 @@ def test_fused_stages_match_unfused_and_oracle(dtype, n, D, fuse):
         ov = torch.vdot(target.reshape(-1), psi.reshape(-1))
         (1 - ov.abs() ** 2).backward()
         out[f] = (to_np(psi), params.grad.cpu().numpy())
-        assert np.abs(out[f][0] - ref).max() < TOL[dtype] * 10
+        assert np.abs(out[f][0] - ref).max() < TOL[dtype]
     g0, gf = out[0][1], out[fuse][1]
     rel = np.linalg.norm(g0 - gf) / np.linalg.norm(g0)
     assert rel < (1e-10 if dtype == C128 else 1e-5), rel
@@ -484,6 +484,35 @@ def test_graph_evaluator_matches_eager_over_replays():
         assert abs(complex(ov) - complex(ref)) < 1e-6
 
 
+def test_evaluator_replay_between_forward_and_backward_keeps_gradients():
+    """A captured evaluation of the same plan with other parameters, run between
+    an autograd forward and its backward, rewrites the shared gate table; the
+    backward must still use the forward's own parameters."""
+    n, D = 4, 6
+    rng = np.random.default_rng(43)
+    ops = circuits.layered(n, 1, rng, sd=0.3)
+    target = torch.as_tensor(circuits.target_state(n, D, 4, rng), dtype=C64, device=DEV)
+    cc = runtime.CompiledCircuit([(o[0], o[1]) for o in ops], n, D, False, C64, DEV)
+    vac = torch.zeros([D] * n, dtype=C64, device=DEV)
+    vac.view(-1)[0] = 1
+    rows = np.zeros((len(ops), 2))
+    for i, (_, _, p) in enumerate(ops):
+        rows[i, : len(p)] = p
+    p = torch.tensor(rows, dtype=torch.float64, device=DEV)
+    q = p * 1.7 + 0.05
+    ref_leaf = p.clone().requires_grad_(True)
+    runtime.fidelity_loss(runtime.CircuitFunction.apply(ref_leaf, vac, cc, None), target).backward()
+    ev = runtime.GradEvaluator(cc, vac, target, use_graph=True)
+    leaf = p.clone().requires_grad_(True)
+    loss = runtime.fidelity_loss(runtime.CircuitFunction.apply(leaf, vac, cc, None), target)
+    _, gq = ev(q)  # replays table build + sweeps with q on the shared plan
+    gq = gq.clone()
+    loss.backward()
+    torch.cuda.synchronize()
+    assert torch.equal(leaf.grad, ref_leaf.grad)
+    assert not torch.equal(gq, ref_leaf.grad)
+
+
 def test_fidelity_loss_gradient_matches_torch():
     rng = np.random.default_rng(3)
     psi = torch.tensor(rng.normal(size=300) + 1j * rng.normal(size=300), dtype=C128, device=DEV, requires_grad=True)
@@ -524,6 +553,35 @@ def test_tf_path_comparator_gradient_matches_adjoint():
     assert float((g1 - g2).norm() / g2.norm()) < 1e-10
 
 
+def test_tf_path_comparator_mixed_with_loss_matches_adjoint():
+    """The TF-path restatement on a density matrix (mix, two-sided gates, loss
+    superoperator differentiated by autograd) against the adjoint kernels, and
+    its forward against the oracle (complex128)."""
+    from strawberryfields_amd import tfpath
+
+    n, D = 2, 5
+    rng = np.random.default_rng(52)
+    ops = circuits.layered(n, 1, rng, sd=0.3, loss=True)
+    ref, _ = fo.run_ops(ops, n, D, pure=False)
+    rows = np.zeros((len(ops), 2))
+    for i, (_, _, p) in enumerate(ops):
+        rows[i, : len(p)] = p
+    idx = torch.as_tensor(np.indices([D] * n).sum(axis=0), dtype=torch.float64, device=DEV)
+    w = (idx - 1.0) ** 2
+    p1 = torch.tensor(rows, dtype=torch.float64, device=DEV, requires_grad=True)
+    tfc = tfpath.TFPathCircuit([(o[0], o[1]) for o in ops], n, D, C128, DEV, mixed=True)
+    rho = tfc.forward(p1)
+    assert np.abs(to_np(rho) - ref).max() < 1e-12
+    (tfc.all_fock_probs(rho) * w).sum().backward()
+    cc = runtime.CompiledCircuit([(o[0], o[1]) for o in ops], n, D, True, C128, DEV)
+    p2 = torch.tensor(rows, dtype=torch.float64, device=DEV, requires_grad=True)
+    rho0 = torch.as_tensor(fo.vacuum(n, D, pure=False), dtype=C128, device=DEV)
+    rho2 = runtime.CircuitFunction.apply(p2, rho0, cc, None)
+    (runtime.all_fock_probs(rho2, n, D, True) * w).sum().backward()
+    g1, g2 = p1.grad, p2.grad
+    assert float((g1 - g2).norm() / g2.norm()) < 1e-10
+
+
 # ------------------------------------------------------------------ per-fiber kernels
 @pytest.mark.parametrize("dtype", [C128, C64])
 @pytest.mark.parametrize("pure,n,D", [(True, 4, 6), (True, 5, 4), (False, 3, 5), (False, 3, 4)])
@@ -540,7 +598,7 @@ def test_per_fiber_kernels_match_oracle_and_fd(dtype, pure, n, D):
     params = torch.tensor(rows, dtype=torch.float64, device=DEV, requires_grad=True)
     st0 = torch.as_tensor(fo.vacuum(n, D, pure=pure), dtype=dtype, device=DEV)
     out = runtime.CircuitFunction.apply(params, st0, cc, None)
-    assert np.abs(to_np(out) - ref).max() < TOL[dtype] * 10
+    assert np.abs(to_np(out) - ref).max() < TOL[dtype]
     idx = np.indices([D] * n).sum(axis=0)
     weights = (idx - 1.0) ** 2
     (runtime.all_fock_probs(out, n, D, not rpure).to(torch.float64) * torch.as_tensor(weights, device=DEV)).sum().backward()
@@ -583,7 +641,7 @@ def test_batch_evaluator_matches_per_circuit_autograd():
             loss.backward()
             assert abs(float(loss) - float(losses[k])) < 1e-5 * max(1.0, abs(float(loss)))
             rel = float((p.grad - grads[k]).norm() / max(p.grad.norm(), 1e-30))
-            assert rel < 1e-4, (k, rel)
+            assert rel < 1e-5, (k, rel)
 
 
 # ------------------------------------------------------------------ LDS slab kernels
@@ -632,7 +690,7 @@ def test_slab_kernels_match_oracle_and_fd(dtype, pure, n, D, pairs):
     params = torch.tensor(rows, dtype=torch.float64, device=DEV, requires_grad=True)
     st0 = torch.as_tensor(fo.vacuum(n, D, pure=pure), dtype=dtype, device=DEV)
     out = runtime.CircuitFunction.apply(params, st0, cc, None)
-    assert np.abs(to_np(out) - ref).max() < TOL[dtype] * 10
+    assert np.abs(to_np(out) - ref).max() < TOL[dtype]
     idx = np.indices([D] * n).sum(axis=0)
     weights = (idx - 1.0) ** 2
     (runtime.all_fock_probs(out, n, D, not rpure).to(torch.float64) * torch.as_tensor(weights, device=DEV)).sum().backward()
@@ -641,7 +699,7 @@ def test_slab_kernels_match_oracle_and_fd(dtype, pure, n, D, pairs):
     sel = np.sort(rng.choice(len(gflat), size=min(8, len(gflat)), replace=False))
     g_fd = _fd_subset(ops, n, D, pure, weights, sel)
     rel = np.linalg.norm(gflat[sel] - g_fd) / np.linalg.norm(g_fd)
-    assert rel < (1e-7 if dtype == C128 else 1e-4), rel
+    assert rel < (1e-8 if dtype == C128 else 1e-5), rel
 
 
 # ------------------------------------------------------------------ c5 structure at a smaller cutoff
@@ -649,8 +707,9 @@ _C5S = {}
 
 
 def _c5_structure_reference():
-    """Oracle forward and central differences (4 seeded parameters) of the
-    c5-shaped circuit at cutoff 10, computed once for both dtypes."""
+    """Oracle forward and Richardson-extrapolated central differences (4 seeded
+    parameters) of the c5-shaped circuit at cutoff 10, computed once for both
+    dtypes."""
     if _C5S:
         return _C5S
     n, D, L = 3, 10, 2
@@ -671,13 +730,16 @@ def _c5_structure_reference():
         st, _ = fo.run_ops(new, n, D, pure=False, state=vac)
         return float((fo.all_fock_probs(st, n, False) * w).sum())
 
-    h = 1e-5
-    g_fd = []
-    for i in sel:
+    # central differences at h and h/2 combined as (4 g(h/2) - g(h)) / 3, which
+    # removes the O(h^2) term: a step small enough for plain differences
+    # leaves ~1e-8 of rounding in the loss (~10) divided by h
+    def central(i, h):
         a, b = flat.copy(), flat.copy()
         a[i] += h
         b[i] -= h
-        g_fd.append((loss_at(a) - loss_at(b)) / (2 * h))
+        return (loss_at(a) - loss_at(b)) / (2 * h)
+
+    g_fd = [(4 * central(i, 5e-4) - central(i, 1e-3)) / 3 for i in sel]
     _C5S.update(n=n, D=D, L=L, ops=ops, vac=vac, ref=ref, w=w, sel=sel, g_fd=np.array(g_fd))
     return _C5S
 
@@ -699,13 +761,13 @@ def test_c5_structure_dm_layers_with_loss_checkpoints(dtype):
     params = torch.tensor(rows, dtype=torch.float64, device=DEV, requires_grad=True)
     rho0 = torch.as_tensor(r["vac"], dtype=dtype, device=DEV)
     rho = runtime.CircuitFunction.apply(params, rho0, cc, len(ops) // r["L"])
-    assert np.abs(to_np(rho) - r["ref"]).max() < TOL[dtype] * 10
+    assert np.abs(to_np(rho) - r["ref"]).max() < TOL[dtype]
     probs = runtime.all_fock_probs(rho, n, D, True).to(torch.float64)
     (probs * torch.as_tensor(r["w"], device=DEV)).sum().backward()
     g = params.grad.cpu().numpy()
     gflat = np.array([g[i, j] for i, (_, _, p) in enumerate(ops) for j in range(len(p))])
     rel = np.linalg.norm(gflat[r["sel"]] - r["g_fd"]) / np.linalg.norm(r["g_fd"])
-    assert rel < (1e-7 if dtype == C128 else 1e-4), rel
+    assert rel < (1e-8 if dtype == C128 else 1e-5), rel
 
 
 # ------------------------------------------------------------------ S2, MZ, CK, V
@@ -833,7 +895,7 @@ def test_new_gates_match_oracle_and_fd(dtype, pure, n, D):
     be = run_backend(ops, n, D, dtype, pure=pure)
     st = be.state()
     got = to_np(st.ket() if rpure else st.dm())
-    assert np.abs(got - ref).max() < TOL[dtype] * 10
+    assert np.abs(got - ref).max() < TOL[dtype]
     cc = runtime.CompiledCircuit([(o[0], o[1]) for o in ops], n, D, not pure, dtype, DEV, fuse=0)
     rows = np.zeros((len(ops), 2))
     for i, (_, _, p) in enumerate(ops):
@@ -841,7 +903,7 @@ def test_new_gates_match_oracle_and_fd(dtype, pure, n, D):
     params = torch.tensor(rows, dtype=torch.float64, device=DEV, requires_grad=True)
     st0 = torch.as_tensor(fo.vacuum(n, D, pure=pure), dtype=dtype, device=DEV)
     out = runtime.CircuitFunction.apply(params, st0, cc, None)
-    assert np.abs(to_np(out) - ref).max() < TOL[dtype] * 10
+    assert np.abs(to_np(out) - ref).max() < TOL[dtype]
     idx = np.indices([D] * n).sum(axis=0)
     weights = (idx - 1.0) ** 2
     probs = runtime.all_fock_probs(out, n, D, not rpure).to(torch.float64)
@@ -852,7 +914,7 @@ def test_new_gates_match_oracle_and_fd(dtype, pure, n, D):
     sel = np.sort(rng.choice(len(gflat), size=min(nsel, len(gflat)), replace=False))
     g_fd = _fd_subset(ops, n, D, pure, weights, sel)
     rel = np.linalg.norm(gflat[sel] - g_fd) / np.linalg.norm(g_fd)
-    assert rel < (1e-7 if dtype == C128 else 1e-4), rel
+    assert rel < (1e-8 if dtype == C128 else 1e-5), rel
 
 
 def test_new_gates_reference_known_answers():
@@ -928,7 +990,7 @@ def test_marginal_projection_and_matrix_kernels(dtype, pure):
     for mode in range(n):
         got = to_np(runtime.apply_matrix(t, n, D, not pure, mode, torch.as_tensor(U, device=DEV)))
         ref = c.apply_gate_BLAS(U, [mode], state=st)
-        assert np.abs(got - ref).max() < TOL[dtype] * 10, mode
+        assert np.abs(got - ref).max() < TOL[dtype], mode
 
 
 @pytest.mark.parametrize("pure", [True, False])
@@ -1052,7 +1114,7 @@ def test_multimode_preparation_matches_oracle(dtype, pure):
             be.prepare_dm_state(state, modes)
         c.prepare_multimode(state, modes)
         assert be._pure == c._pure, modes
-        assert np.abs(to_np(be._state) - c._state).max() < TOL[dtype] * 10, modes
+        assert np.abs(to_np(be._state) - c._state).max() < TOL[dtype], modes
 
 
 def test_engine_collects_measurement_samples():
@@ -1102,7 +1164,7 @@ def test_interferometer_matches_exact_passive_action(dtype, pure, mesh):
         ref = fo.passive_fock_action(P @ U @ P.T, psi, n, D)
         got = to_np(res.state.ket() if pure else res.state.dm())
         want = ref if pure else fo.mix(ref, n)
-        assert np.abs(got - want).max() < TOL[dtype] * 10, order
+        assert np.abs(got - want).max() < TOL[dtype], order
 
 
 @pytest.mark.parametrize("dtype", [C128, C64])
@@ -1126,7 +1188,7 @@ def test_diagonal_runs_match_oracle_and_fd(dtype, pure, n, D):
     params = torch.tensor(rows, dtype=torch.float64, device=DEV, requires_grad=True)
     st0 = torch.as_tensor(fo.vacuum(n, D, pure=pure), dtype=dtype, device=DEV)
     out = runtime.CircuitFunction.apply(params, st0, cc, None)
-    assert np.abs(to_np(out) - ref).max() < TOL[dtype] * 10
+    assert np.abs(to_np(out) - ref).max() < TOL[dtype]
     idx = np.indices([D] * n).sum(axis=0)
     weights = (idx - 1.0) ** 2
     (runtime.all_fock_probs(out, n, D, not rpure).to(torch.float64) * torch.as_tensor(weights, device=DEV)).sum().backward()
@@ -1191,7 +1253,7 @@ def test_maximum_cutoff_tables_and_ops(dtype):
         st32 = rand_state(1, 32, pure, 32)
         got = run_backend(ops, 1, 32, dtype, pure=pure, init=st32).state()
         ref, rpure = fo.run_ops(ops, 1, 32, state=st32)
-        assert np.abs(to_np(got.ket() if rpure else got.dm()) - ref).max() < TOL[dtype] * 10, ("V", pure)
+        assert np.abs(to_np(got.ket() if rpure else got.dm()) - ref).max() < TOL[dtype], ("V", pure)
         st2 = rand_state(2, 40, pure, 33)
         for name, p in [("BS", (0.6, 0.4)), ("S2", (0.2, 0.5)), ("CK", (0.4,))]:
             for modes in [(0, 1), (1, 0)]:
@@ -1225,7 +1287,7 @@ def test_single_mode_preparations_match_closed_forms(dtype):
         al, z = rd * np.exp(1j * pd), rs * np.exp(1j * ps)
         return (expm(al * a.T - np.conj(al) * a) @ expm(0.5 * (np.conj(z) * a @ a - z * a.T @ a.T)) @ vac)[:D]
 
-    tol = TOL[dtype] * 10
+    tol = TOL[dtype]
     cases = [
         ("prepare_coherent_state", (0.7, 0.4), dsq(0.7, 0.4, 0, 0), tol),
         ("prepare_squeezed_state", (0.5, -0.3), dsq(0, 0, 0.5, -0.3), tol),

@@ -567,3 +567,14 @@ def test_bench_refuses_more_than_one_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode != 0 and "one GPU" in (r.stdout + r.stderr)
+
+
+def test_batch_size_option_is_rejected_clearly():
+    """The TF backend's batch_size axis (tfbackend/backend.py:84-106) is not
+    offered: begin_circuit raises NotImplementedError instead of silently
+    running unbatched."""
+    from strawberryfields_amd import HIPFockBackend
+
+    be = HIPFockBackend()
+    with pytest.raises(NotImplementedError, match="batch_size"):
+        be.begin_circuit(2, cutoff_dim=4, batch_size=3)

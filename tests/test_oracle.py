@@ -163,6 +163,27 @@ def test_beamsplitter_vs_kronecker_both_orders(faithful):
         assert np.abs(cm._state - fo.mix(ref, n)).max() < 1e-10, (m1, m2, "mixed")
 
 
+@pytest.mark.parametrize("rule", ["_two_mode_passive", "_two_mode_diffcons"])
+def test_native_selection_rule_loops_match_the_python_loops(rule):
+    """oracle/fock_cpu.c (the faithful mode's stand-in for the reference's numba
+    loops) against the same loops in Python, on random matrices and states with
+    trailing axes; both are pinned against the Kronecker operator above."""
+    if fo.native_lib() is None:
+        pytest.skip("oracle/libfock_cpu.so not built")
+    rng = np.random.default_rng(13)
+    for D, rest in [(1, 3), (2, 1), (5, 7), (8, 16)]:
+        mat = rng.normal(size=[D] * 4) + 1j * rng.normal(size=[D] * 4)
+        st = rng.normal(size=(D, D, rest)) + 1j * rng.normal(size=(D, D, rest))
+        got = getattr(fo.Circuit, rule)(mat, st, D, True)
+        saved = fo._NATIVE["lib"]
+        try:
+            fo._NATIVE["lib"] = None
+            want = getattr(fo.Circuit, rule)(mat, st, D, True)
+        finally:
+            fo._NATIVE["lib"] = saved
+        assert np.abs(got - want).max() < 1e-12 * max(1.0, np.abs(want).max()), (rule, D, rest)
+
+
 def test_faithful_and_fast_agree_on_config1():
     from strawberryfields_amd import circuits
 
