@@ -7,6 +7,7 @@ object under ``_build/`` and the objects are linked into
 """
 import concurrent.futures
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -50,27 +51,41 @@ def _compile(src, obj, verbose):
     return obj
 
 
-def _stale(src, obj, headers):
+_INCLUDE = re.compile(r'^\s*#\s*include\s+"([^"]+)"', re.M)
+
+
+def _deps(path, seen=None):
+    """The file and every header it includes with quotes, transitively."""
+    seen = set() if seen is None else seen
+    path = os.path.normpath(path)
+    if path in seen or not os.path.exists(path):
+        return seen
+    seen.add(path)
+    with open(path) as f:
+        for inc in _INCLUDE.findall(f.read()):
+            _deps(os.path.join(os.path.dirname(path), inc), seen)
+    return seen
+
+
+def _stale(src, obj):
     if not os.path.exists(obj):
         return True
     t = os.path.getmtime(obj)
-    return any(os.path.getmtime(f) > t for f in [src] + headers)
+    return any(os.path.getmtime(f) > t for f in _deps(src))
 
 
 def build(verbose=False, jobs=8, changed_only=False):
     """Compile every HIP source and link libsfamd.so; returns the library path.
 
-    changed_only: recompile only sources newer than their object (or than any
-    header); the default rebuilds everything."""
+    changed_only: recompile only sources older than their object or than a
+    header they include; the default rebuilds everything."""
     os.makedirs(BUILD_DIR, exist_ok=True)
     srcs = sorted(f for f in os.listdir(CSRC) if f.endswith(".hip"))
     objs = [os.path.join(BUILD_DIR, f[:-4] + ".o") for f in srcs]
-    headers = [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".h")]
-    headers.append(os.path.join(REPO_DIR, "include", "sfamd.h"))
     todo = [
         (os.path.join(CSRC, s), o)
         for s, o in zip(srcs, objs)
-        if not changed_only or _stale(os.path.join(CSRC, s), o, headers)
+        if not changed_only or _stale(os.path.join(CSRC, s), o)
     ]
     with concurrent.futures.ThreadPoolExecutor(max_workers=jobs) as ex:
         futs = [ex.submit(_compile, s, o, verbose) for s, o in todo]

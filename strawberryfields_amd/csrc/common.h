@@ -233,12 +233,16 @@ int loss_adj(const void* psi, const void* lam_out, void* lam_in, int64_t N,
              double* partial, int nwg, int zp1, int dtype, hipStream_t s);
 int mix_adj(const void* ket, const void* g_dm, void* g_ket, int n, int D,
             int dtype, hipStream_t s);
-// grads[2*op + slot] = sum over passes p<2 and w<nwg of
-// partial[((op*2 + p)*2 + slot)*nwg + w], for the listed ops
-constexpr int kOpListMax = 960;
+// grads[2*op + slot] = sum over passes p < npass and columns w < ncol of
+// partial[((op*2 + p)*2 + slot)*nwg + w], for the listed ops; cols[i] holds
+// ncol (<= nwg) with bit 15 set when only pass 0 was written (fused stages
+// write one column per launched workgroup)
+constexpr int kOpListMax = 600;  // fits the 4 KB kernel-argument space
+constexpr unsigned kOpColsOnePass = 0x8000u;
 struct OpList {
   int32_t n;
   int32_t ops[kOpListMax];
+  uint16_t cols[kOpListMax];
 };
 int reduce_grads(const double* partial, double* grads, const OpList& ol, int nwg, hipStream_t s);
 

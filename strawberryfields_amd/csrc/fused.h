@@ -20,8 +20,8 @@ struct StageGate {
   int8_t pad;
   int32_t op;     // program operation index (gradient slot)
   int64_t off;    // gate-table offset, complex elements
-  int32_t moff;   // LDS offset of the matrix, forward layout (matrix only)
-  int32_t amoff;  // LDS offset of matrix + derivatives, adjoint layout
+  int32_t moff;   // LDS offset of the gate's matrices (complex elements)
+  int32_t pad2;
 };
 
 struct StageArgs {
@@ -35,20 +35,42 @@ struct StageArgs {
   StageGate g[kStageMaxGates];
 };
 
-// threads per workgroup of the stage kernels: one tile element per thread up
-// to 1024 (more waves in flight per SIMD to hide LDS latency), at least 256
-__host__ __device__ constexpr int stage_threads(int DK) {
-  return DK >= 1024 ? 1024 : (DK <= 256 ? 256 : ((DK + 63) / 64) * 64);
+// Work decomposition of one gate on a tile of K local modes: the tile's D^K
+// amplitudes form F = D^(K-1) fibers of D amplitudes (along the gate's axis;
+// for a beamsplitter, the modular anti-diagonals of its two axes), and each
+// work item computes `rows` outputs of one fiber from the whole fiber.
+__host__ __device__ constexpr int stage_rows(int D) { return D <= 10 ? 2 : 4; }
+__host__ __device__ constexpr int stage_items(int D, int K) {
+  return (D == 0 ? 0 : (K == 2 ? D : K == 3 ? D * D : D * D * D)) * ((D + stage_rows(D) - 1) / stage_rows(D));
+}
+// threads per workgroup: one per work item, 64 to 1024
+__host__ __device__ constexpr int stage_threads(int D, int K) {
+  return stage_items(D, K) >= 1024 ? 1024 : (stage_items(D, K) <= 64 ? 64 : ((stage_items(D, K) + 63) / 64) * 64);
+}
+// LDS tile: one padding slot after every D amplitudes (bank spread)
+__host__ __device__ constexpr int64_t stage_tile_pad(int D, int K) {
+  return (K == 2 ? D * D : K == 3 ? D * D * D : D * D * D * D) + (K == 2 ? D : K == 3 ? D * D : D * D * D);
 }
 
 bool stage_supported(int D, int K);
-// matrix + derivative table elements of one gate (the gate-table entry)
+// LDS matrix elements of one gate: forward (the matrix) or adjoint (matrix and
+// the first parameter's derivative), plus D slack elements
 int64_t stage_gate_elems(int kind, int D, bool derivs);
 // dynamic LDS bytes the stage kernels need
 int64_t stage_lds_fwd(int D, int K, int mat_fwd, int dtype);
 int64_t stage_lds_adj(int D, int K, int ngates, int mat_adj, int dtype);
 int stage_fwd(int D, int K, const void* in, void* out, const void* table, const StageArgs& a, int grid, int dtype,
               hipStream_t s);
+// per-cutoff launchers behind stage_fwd / stage_adj (fused_dNN_MM.hip)
+#define SFAMD_STAGE_FOR_EACH_D(X) \
+  X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18) X(19) X(20)
+#define SFAMD_STAGE_DECLARE_D(d)                                                                                 \
+  int stage_fwd_d##d(int K, const void* in, void* out, const void* table, const StageArgs& a, int grid, int dtype, \
+                     hipStream_t s);                                                                             \
+  int stage_adj_d##d(int K, const void* psi, const void* lo, void* li, const void* table, const StageArgs& a,      \
+                     int grid, double* partial, int nwg, int dtype, hipStream_t s);
+SFAMD_STAGE_FOR_EACH_D(SFAMD_STAGE_DECLARE_D)
+#undef SFAMD_STAGE_DECLARE_D
 int stage_adj(int D, int K, const void* psi, const void* lam_out, void* lam_in, const void* table,
               const StageArgs& a, int grid, double* partial, int nwg, int dtype, hipStream_t s);
 

@@ -539,10 +539,12 @@ __global__ void __launch_bounds__(256)
 k_reduce_grads(const double* __restrict__ partial, double* __restrict__ grads, sfamd::OpList ol, int nwg) {
   __shared__ double sh[8];
   const int op = ol.ops[blockIdx.x / 2], slot = blockIdx.x % 2;
+  const unsigned c = ol.cols[blockIdx.x / 2];
+  const int ncol = (int)(c & ~sfamd::kOpColsOnePass), npass = (c & sfamd::kOpColsOnePass) ? 1 : 2;
   double v = 0.0;
-  for (int p = 0; p < 2; ++p) {
+  for (int p = 0; p < npass; ++p) {
     const double* src = partial + (((int64_t)op * 2 + p) * 2 + slot) * nwg;
-    for (int w = threadIdx.x; w < nwg; w += blockDim.x) v += src[w];
+    for (int w = threadIdx.x; w < ncol; w += blockDim.x) v += src[w];
   }
   const double s = block_sum(v, sh);
   if (threadIdx.x == 0) grads[2 * op + slot] = s;

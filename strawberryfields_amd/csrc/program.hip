@@ -338,16 +338,16 @@ unsigned op_mask(const OpPlan& op) {
 // if its modes lie in the mask, no earlier op left outside the stage touches
 // one of them (gates on disjoint modes commute) and the adjoint kernel's LDS
 // (one tile per gate plus two, and every gate's matrix + derivatives) fits
-void absorb(const std::vector<OpPlan>& ops, const std::vector<int>& rem, unsigned mask, int D, int64_t tile_bytes,
-            int64_t esz, std::vector<int>* got) {
+void absorb(const std::vector<OpPlan>& ops, const std::vector<int>& rem, unsigned mask, int D, int Kc,
+            int64_t tile_bytes, int64_t esz, std::vector<int>* got) {
   got->clear();
   unsigned blocked = 0;
-  int64_t used = 2 * tile_bytes;
+  int64_t used = 3 * tile_bytes;  // the stage input and two cotangent tiles
   for (int i : rem) {
     const OpPlan& op = ops[i];
     const unsigned om = op_mask(op);
-    const int64_t cost = tile_bytes + stage_gate_elems(op.kind, D, true) * esz +
-                         2 * (stage_threads((int)(tile_bytes / esz)) / 64) * 8;
+    const int64_t cost = tile_bytes + (((stage_gate_elems(op.kind, D, true) + 1) & ~(int64_t)1) * esz) +
+                         2 * (stage_threads(D, Kc) / 64) * 8;
     if ((int)got->size() < kStageMaxGates && op_in(op, mask) && !(om & blocked) && used + cost <= kStageLdsBudget) {
       got->push_back(i);
       used += cost;
@@ -361,7 +361,7 @@ void absorb(const std::vector<OpPlan>& ops, const std::vector<int>& rem, unsigne
 void plan_run(sfamd_prog* p, const std::vector<int>& run, int Kc) {
   const int n = p->n, D = p->D;
   const int64_t esz = elem_size(p->dtype);
-  const int64_t tile_bytes = ipow(D, Kc) * esz;
+  const int64_t tile_bytes = stage_tile_pad(D, Kc) * esz;
   std::vector<int> rem = run;
   std::vector<unsigned> cands;
   // all Kc-subsets when few, else subsets grown from the next ops
@@ -397,7 +397,7 @@ void plan_run(sfamd_prog* p, const std::vector<int>& run, int Kc) {
     unsigned bm = 0;
     long bkey = -1;
     for (unsigned m : local) {
-      absorb(p->ops, rem, m, D, tile_bytes, esz, &got);
+      absorb(p->ops, rem, m, D, Kc, tile_bytes, esz, &got);
       // more gates first; then tiles that include the last modes (contiguous runs)
       const long key = (long)got.size() * (1L << 26) + (long)m;
       if (key > bkey) {
@@ -426,7 +426,7 @@ void plan_run(sfamd_prog* p, const std::vector<int>& run, int Kc) {
     a.nnl = nlc;
     a.ntiles = ipow(D, nlc);
     for (int t = 0; t < Kc; ++t) a.loc_stride[t] = ipow(D, n - 1 - lmodes[t]);
-    int64_t mf = 0, ma = 0;
+    int64_t ma = 0;
     for (int gi = 0; gi < a.ngates; ++gi) {
       const OpPlan& op = p->ops[best[gi]];
       StageGate& g = a.g[gi];
@@ -438,13 +438,12 @@ void plan_run(sfamd_prog* p, const std::vector<int>& run, int Kc) {
       }
       g.op = best[gi];
       g.off = op.off;
-      g.moff = (int32_t)mf;
-      g.amoff = (int32_t)ma;
-      // keep each matrix 16-byte aligned in LDS
-      mf += (stage_gate_elems(op.kind, D, false) + 1) & ~(int64_t)1;
+      // one region per gate in both kernels (the forward fills its first
+      // part); keep each region 16-byte aligned in LDS
+      g.moff = (int32_t)ma;
       ma += (stage_gate_elems(op.kind, D, true) + 1) & ~(int64_t)1;
     }
-    a.mat_fwd = (int32_t)mf;
+    a.mat_fwd = (int32_t)ma;
     a.mat_adj = (int32_t)ma;
     p->steps.push_back(st);
     std::vector<int> nrem;
@@ -466,7 +465,7 @@ void plan_steps(sfamd_prog* p) {
   // fusion needs compiled stage kernels and room for at least one gate of any kind
   int gmax = 0;
   if (Kc >= 2 && n <= kStageMaxModes && stage_supported(D, Kc)) {
-    const int64_t one = 4 * ipow(D, Kc) * esz + stage_gate_elems(SFAMD_OP_BEAMSPLITTER, D, true) * esz + 256;
+    const int64_t one = 4 * stage_tile_pad(D, Kc) * esz + stage_gate_elems(SFAMD_OP_BEAMSPLITTER, D, true) * esz + 256;
     gmax = one <= kStageLdsBudget ? 1 : 0;
   }
   std::vector<int> run;
@@ -801,8 +800,7 @@ extern "C" int sfamd_prog_adjoint(const sfamd_prog_t* p, int begin, int end, con
     void* dst = (i == begin) ? g_in : ((lam == LA) ? LB : LA);
     const Step& st = p->steps[i];
     if (st.fused) {
-      // one workgroup per tile at most; the kernel clears the partial columns
-      // of the groups it does not launch
+      // one workgroup per tile at most, one partial column per workgroup
       const int grid = (int)(st.args.ntiles < nwg ? st.args.ntiles : nwg);
       RET_IF(stage_adj(p->D, st.K, saved[i - begin], lam, dst, table, st.args, grid, partial, nwg, p->dtype, s));
     } else if (st.diag) {
@@ -817,7 +815,10 @@ extern "C" int sfamd_prog_adjoint(const sfamd_prog_t* p, int begin, int end, con
   OpList ol;
   ol.n = 0;
   for (int i = begin; i < end; ++i) {
-    for (int op : p->steps[i].ops) {
+    const Step& st = p->steps[i];
+    const int ncol = st.fused ? (int)(st.args.ntiles < nwg ? st.args.ntiles : nwg) : nwg;
+    for (int op : st.ops) {
+      ol.cols[ol.n] = (uint16_t)(st.fused ? ((unsigned)ncol | kOpColsOnePass) : (unsigned)ncol);
       ol.ops[ol.n++] = op;
       if (ol.n == kOpListMax) {
         RET_IF(reduce_grads(partial, grads, ol, nwg, s));
