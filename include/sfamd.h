@@ -17,7 +17,8 @@
  *   - `stream` is a hipStream_t passed as void*; NULL is the null stream.
  *     All work is enqueued asynchronously on that stream.
  *   - Buffers are device pointers owned by the caller.  The library never
- *     allocates device memory; scratch space is caller-provided.
+ *     allocates device memory; scratch space is caller-provided (except for
+ *     sfamd_fidelity_loss_noalloc, whose scratch is a static per-GPU array).
  *   - `dtype` selects the amplitude type: SFAMD_C64 (interleaved float re,im)
  *     or SFAMD_C128 (interleaved double re,im).
  *   - A state of `n` modes and cutoff `D` is a ket of n axes [D]*n in C order
@@ -196,6 +197,18 @@ int sfamd_overlap(const void* a, const void* b, int64_t n_elems, double* partial
 int sfamd_fidelity_loss(const void* target, const void* psi, int64_t n_elems,
                         double* partial, double* out_dev, void* g_psi,
                         int dtype, void* stream);
+
+/* Allocation-free variant of sfamd_fidelity_loss; no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * partial sums live in a file-scope device array of
+ * SFAMD_NOALLOC_SCRATCH_DOUBLES doubles per GPU, held until the process ends
+ * (n_elems whose sfamd_overlap_partials exceed it return SFAMD_E_WORKSPACE).
+ * The device whose array is used is the current device, which must be the
+ * stream's.  Calls that overlap on one GPU share that array and race. */
+#define SFAMD_NOALLOC_SCRATCH_DOUBLES 2048
+int sfamd_fidelity_loss_noalloc(const void* target, const void* psi, int64_t n_elems,
+                                double* out_dev, void* g_psi, int dtype, void* stream);
 
 /* Total mean photon number of a ket, L = sum_I |psi_I|^2 sum_m I_m (FP64,
  * out_dev[0]), and if g_psi != NULL its cotangent g = 2 (sum_m I_m) psi.

@@ -101,6 +101,7 @@ SIGNATURES = {
     "sfamd_overlap_partials": (_I64, [_I64]),
     "sfamd_overlap": (_I, [_P, _P, _I64, _P, _P, _I, _P]),
     "sfamd_fidelity_loss": (_I, [_P, _P, _I64, _P, _P, _P, _I, _P]),
+    "sfamd_fidelity_loss_noalloc": (_I, [_P, _P, _I64, _P, _P, _I, _P]),
     "sfamd_number_loss": (_I, [_P, _P, _I, _I, _P, _P, _I, _P]),
     "sfamd_marginal_partials": (_I64, [_I, _I, _I]),
     "sfamd_marginal_probs": (_I, [_P, _I, _I, _I, ctypes.POINTER(_I), _I, _P, _P, _I, _P]),
