@@ -21,7 +21,12 @@ LIB_PATH = os.path.join(PKG_DIR, "libsfamd.so")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 
 # -packed-fp32-ops: the device code is built without packed FP32 instructions
-# (v_pk_*_f32); the host pass warns that it does not know the feature
+# (v_pk_*_f32); the host pass warns that it does not know the feature.
+# -dot7-insts: without the byte dot-product instructions.  With them, the
+# gfx950 backend rewrote an index of the form e - k*s0 + k*s1 (k, s0, s1 known
+# to fit in a byte) into v_perm_b32 + v_dot4_u32_u8, which adds k*s0 instead
+# of subtracting it: the fused beamsplitter adjoint at cutoffs 12-15 read the
+# wrong cotangent amplitudes.  No kernel here uses dot-product instructions.
 HIP_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
@@ -31,6 +36,10 @@ HIP_FLAGS = [
     "-target-feature",
     "-Xclang",
     "-packed-fp32-ops",
+    "-Xclang",
+    "-target-feature",
+    "-Xclang",
+    "-dot7-insts",
     "-I" + os.path.join(REPO_DIR, "include"),
 ]
 
@@ -40,9 +49,11 @@ def _compile(src, obj, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
-    # the host pass reports the device-only target feature as unknown; harmless
+    # the host pass reports the device-only target features as unknown; harmless
     err = "\n".join(
-        l for l in res.stderr.splitlines() if "packed-fp32-ops" not in l and l.strip()
+        l
+        for l in res.stderr.splitlines()
+        if "packed-fp32-ops" not in l and "dot7-insts" not in l and l.strip()
     )
     if res.returncode != 0:
         raise RuntimeError("hipcc failed for %s:\n%s" % (src, err))

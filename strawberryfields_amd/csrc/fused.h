@@ -20,8 +20,8 @@ struct StageGate {
   int8_t pad;
   int32_t op;     // program operation index (gradient slot)
   int64_t off;    // gate-table offset, complex elements
-  int32_t moff;   // LDS offset of the gate's matrices (complex elements)
-  int32_t pad2;
+  int32_t moff;   // LDS offset of the matrix, forward layout (matrix only)
+  int32_t amoff;  // LDS offset of matrix + derivative, adjoint layout
 };
 
 struct StageArgs {
@@ -35,26 +35,19 @@ struct StageArgs {
   StageGate g[kStageMaxGates];
 };
 
-// Work decomposition of one gate on a tile of K local modes: the tile's D^K
-// amplitudes form F = D^(K-1) fibers of D amplitudes (along the gate's axis;
-// for a beamsplitter, the modular anti-diagonals of its two axes), and each
-// work item computes `rows` outputs of one fiber from the whole fiber.
-__host__ __device__ constexpr int stage_rows(int D) { return D <= 10 ? 2 : 4; }
-__host__ __device__ constexpr int stage_items(int D, int K) {
-  return (D == 0 ? 0 : (K == 2 ? D : K == 3 ? D * D : D * D * D)) * ((D + stage_rows(D) - 1) / stage_rows(D));
+// tile of K local modes: D^K amplitudes
+__host__ __device__ constexpr int stage_tile(int D, int K) {
+  return K == 2 ? D * D : K == 3 ? D * D * D : D * D * D * D;
 }
-// threads per workgroup: one per work item, 64 to 1024
+// threads per workgroup: one tile amplitude per thread up to 1024, at least
+// 256 (more waves in flight per SIMD to hide LDS latency)
 __host__ __device__ constexpr int stage_threads(int D, int K) {
-  return stage_items(D, K) >= 1024 ? 1024 : (stage_items(D, K) <= 64 ? 64 : ((stage_items(D, K) + 63) / 64) * 64);
-}
-// LDS tile: one padding slot after every D amplitudes (bank spread)
-__host__ __device__ constexpr int64_t stage_tile_pad(int D, int K) {
-  return (K == 2 ? D * D : K == 3 ? D * D * D : D * D * D * D) + (K == 2 ? D : K == 3 ? D * D : D * D * D);
+  return stage_tile(D, K) >= 1024 ? 1024 : (stage_tile(D, K) <= 256 ? 256 : ((stage_tile(D, K) + 63) / 64) * 64);
 }
 
 bool stage_supported(int D, int K);
 // LDS matrix elements of one gate: forward (the matrix) or adjoint (matrix and
-// the first parameter's derivative), plus D slack elements
+// the first parameter's derivative), in the gate table's compact layout
 int64_t stage_gate_elems(int kind, int D, bool derivs);
 // dynamic LDS bytes the stage kernels need
 int64_t stage_lds_fwd(int D, int K, int mat_fwd, int dtype);

@@ -11,34 +11,28 @@ bool stage_supported(int D, int K) {
 }
 
 int64_t stage_gate_elems(int kind, int D, bool derivs) {
-  int64_t e;
   switch (kind) {
     case SFAMD_OP_ROTATION:
     case SFAMD_OP_KERR:
-      e = D;
-      break;
+      return D;
     case SFAMD_OP_DISPLACEMENT:
     case SFAMD_OP_SQUEEZE:
-      e = derivs ? 2 * (int64_t)D * D : (int64_t)D * D;
-      break;
+      return derivs ? 2 * (int64_t)D * D : (int64_t)D * D;
     case SFAMD_OP_BEAMSPLITTER:
-      if (D <= kFullBsMaxD) return (derivs ? 2 : 1) * ipow(D, 3);  // per-fiber layout
-      e = derivs ? 2 * bs_nb(D) : bs_nb(D);
-      break;
+      return derivs ? 2 * bs_nb(D) : bs_nb(D);
     default:
       return 0;
   }
-  return e + D;  // D zeros after the gate's tables
 }
 
 int64_t stage_lds_fwd(int D, int K, int mat_fwd, int dtype) {
   const int64_t esz = dtype == SFAMD_C64 ? 8 : 16;
-  return (2 * stage_tile_pad(D, K) + mat_fwd) * esz;
+  return (2 * (int64_t)stage_tile(D, K) + mat_fwd) * esz;
 }
 
 int64_t stage_lds_adj(int D, int K, int ngates, int mat_adj, int dtype) {
   const int64_t esz = dtype == SFAMD_C64 ? 8 : 16;
-  return ((ngates + 3) * stage_tile_pad(D, K) + mat_adj) * esz +
+  return ((ngates + 3) * (int64_t)stage_tile(D, K) + mat_adj) * esz +
          (int64_t)ngates * 2 * (stage_threads(D, K) / 64) * 8;
 }
 

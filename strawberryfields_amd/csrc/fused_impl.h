@@ -10,31 +10,29 @@
 // the planner (program.hip) may move a gate into an earlier stage as long as
 // no earlier gate shares a mode with it.
 //
-// Work decomposition of a gate: the tile's D^K amplitudes form F = D^(K-1)
-// fibers of D amplitudes, and a work item computes R output rows of one fiber
-// from the whole fiber, held in registers, so each LDS read of the fiber
-// feeds R outputs.
-//  * single-mode gate on local axis a: the fibers run along a; the items of
-//    one wave share their rows, so the matrix reads are wave-wide broadcasts;
-//  * beamsplitter on local axes (a0, a1): photon number s = n0 + n1 is
-//    conserved, and fiber (z, p) collects the modular anti-diagonal
-//    {(n0 = k, n1 = (p - k) mod D)}: the full block s = p (k <= p) and the
-//    partial block s = p + D (k > p).  The gate acts on each fiber as
-//    blockdiag(B_p, B_(p+D)); a row of B_p reads the fiber with its k > p
-//    entries zeroed, a row of B_(p+D) the opposite, so every item runs the same
-//    unrolled D-term loop over the compact block table (out-of-block table
-//    reads are finite and multiply zeros).
-// The LDS tile has one padding slot after every D amplitudes, which spreads
-// the fibers of the innermost axis (D amplitudes apart) over the banks.
+// Each thread owns one tile amplitude (or a few, for tiles above 1024) for the
+// whole kernel; its digits on the K local axes are computed once, and a gate
+// on runtime (workgroup-uniform) axes picks the digits it needs.  A gate's
+// output amplitude reads its fiber from the LDS tile and the matrix row from
+// the LDS matrix region: R and K one term, D a row of D terms, S the terms of
+// the same parity, BS the photon-number block s = n0 + n1 of the compact
+// block table.  One small code path per gate kind keeps the instruction
+// footprint small (the stage kernels run once per stage on small states).
+//
+// The stage prologue issues the tile's global loads first, then every gate
+// matrix load of the stage (batched into registers), and only then waits, so
+// a stage pays about one memory latency before its first gate.
 //
 // The adjoint of a stage loads the stage's input tile (its checkpoint) and the
 // incoming cotangent tile, recomputes the stage forward keeping every gate's
-// input and output tile in LDS, and runs the gates backward:
-//   lam_in = G^H lam_out,  mu = (dG/dp0)^H lam_out,  g0 = Re sum conj(mu) psi_in,
+// input and output tile in LDS, and runs the gates backward with each thread's
+// amplitude taken as an INPUT index k of the gate:
+//   lam_in[k] = sum_i conj(G[i][k]) lam_out[i],  mu[k] = sum_i conj(dG/dp0[i][k]) lam_out[i],
+//   g0 += Re conj(mu[k]) psi_in[k],
 // and the phase parameter from the identity dG/dphi = i f (out - in) G
 // (D: f = 1 on the mode's digit, S: f = 1/2; BS: i (n - N) on the first mode),
-//   g1 = f Re i [sum_m m conj(lam_out,m) psi_out,m - sum_n n conj(lam_in,n) psi_in,n],
-// which needs no second matrix product.  R and K: g = Re sum conj(lam) i w f psi.
+//   g1 += f k [Im conj(lam_in[k]) psi_in[k] - Im conj(lam_out[k]) psi_out[k]],
+// which needs no second matrix product.  R and K: g = Re sum conj(lam) i w psi.
 // Parameter sums are reduced per wavefront by DPP lane moves and per
 // workgroup in a fixed order (deterministic), one partial per workgroup.
 //
@@ -62,19 +60,24 @@ struct Pow<D, 0> {
 template <int D, int K>
 struct Geo {
   static constexpr int DK = Pow<D, K>::v;
-  static constexpr int F = DK / D;  // fibers of one axis
-  static constexpr int PDK = DK + F;  // padded LDS tile
-  static constexpr int R = sfamd::stage_rows(D);
-  static constexpr int G = (D + R - 1) / R;  // row groups
-  static constexpr int ITEMS = F * G;
   static constexpr int NT = sfamd::stage_threads(D, K);
+  static constexpr int PER = (DK + NT - 1) / NT;  // tile amplitudes per thread
 };
 
-// padded LDS position of tile element e
-template <int D>
-__device__ __forceinline__ int pad(int e) {
-  return e + e / D;
+// tile stride of local axis a (0 = slowest): D^(K-1-a)
+template <int D, int K>
+__device__ __forceinline__ int axis_stride(int a) {
+  int s = 1;
+#pragma unroll
+  for (int t = 0; t < K - 1; ++t)
+    if (t >= a) s *= D;
+  return s;
 }
+
+// digit of the thread's amplitude on the (uniform, runtime) axis a: the K
+// digits are packed 8 bits each (axis t at bit 8 t), so the pick is one
+// bitfield extract with a scalar offset
+__device__ __forceinline__ int pick(unsigned packed, int a) { return (int)__builtin_amdgcn_ubfe(packed, 8 * a, 8); }
 
 // global offset of tile element e relative to the tile base
 template <int D, int K>
@@ -100,16 +103,48 @@ __device__ __forceinline__ int64_t tile_base(int64_t t, const sfamd::StageArgs& 
   return base;
 }
 
-// ---- staging of a stage's gate tables into LDS (once per workgroup) -------
-// Each gate region holds the matrix (forward) or the matrix and its first
-// parameter's derivative (adjoint).  Beamsplitters at D <= kFullBsMaxD are
-// expanded to the zero-padded per-fiber layout M[r][k][p] (row r, column k of
-// the fiber matrix blockdiag(B_p, B_(p+D)) of anti-diagonal p), so the lanes of
-// a wave (consecutive p) read consecutive entries and no masking is needed;
-// at larger D the compact blocks are kept and followed by D zeros (their rows
-// read up to D - 1 entries past a block).
-constexpr int kFullBsMaxD = 10;
+// The thread's amplitudes e = threadIdx.x + j NT: their local digits and their
+// global offsets within a tile, fixed for the kernel.
+template <int D, int K>
+struct Mine {
+  unsigned dg[Geo<D, K>::PER];  // digit of axis t at bits 8 t
+  int64_t goff[Geo<D, K>::PER];
+  __device__ __forceinline__ Mine(const sfamd::StageArgs& a) {
+#pragma unroll
+    for (int j = 0; j < Geo<D, K>::PER; ++j) {
+      int r = threadIdx.x + j * Geo<D, K>::NT;
+      unsigned p = 0;
+#pragma unroll
+      for (int t = K - 1; t >= 0; --t) {
+        const int q = r / D;
+        p |= (unsigned)(r - q * D) << (8 * t);
+        r = q;
+      }
+      dg[j] = p;
+      goff[j] = local_offset<D, K>(threadIdx.x + j * Geo<D, K>::NT, a);
+    }
+  }
+};
 
+template <int D, int K>
+__device__ __forceinline__ bool owns(int j) {
+  return Geo<D, K>::DK % Geo<D, K>::NT == 0 || (int)threadIdx.x + j * Geo<D, K>::NT < Geo<D, K>::DK;
+}
+
+// the thread's amplitudes of the tile at global offset b, into registers
+template <typename T, int D, int K>
+__device__ __forceinline__ void fetch_tile(const C<T>* __restrict__ src, int64_t b, const Mine<D, K>& me,
+                                           C<T> (&v)[Geo<D, K>::PER]) {
+#pragma unroll
+  for (int j = 0; j < Geo<D, K>::PER; ++j)
+    if (owns<D, K>(j)) v[j] = src[b + me.goff[j]];
+}
+
+// ---- staging of a stage's gate tables into LDS (once per workgroup) -------
+// A gate's LDS region holds the head of its gate-table entry: the matrix
+// (forward) or the matrix and its first parameter's derivative (adjoint), in
+// the table's compact layout (R, K: the diagonal; D, S: row-major D x D; BS:
+// the photon-number blocks).
 template <int D>
 __device__ __forceinline__ int gate_table_elems(int kind, bool derivs) {
   switch (kind) {
@@ -126,47 +161,36 @@ __device__ __forceinline__ int gate_table_elems(int kind, bool derivs) {
   }
 }
 
-// compact-table row bases of fiber p: row r of the fiber reads
-// table[row_base(r) + k] for its block's columns k
-struct BsRows {
-  int o1, c1, o2, c2;
-};
-template <int D>
-__device__ __forceinline__ BsRows bs_rows(int p) {
-  BsRows b;
-  b.c1 = p + 1;
-  b.c2 = D - 1 - p;
-  b.o1 = (int)bs_block_off(p, D);
-  b.o2 = (int)bs_block_off(p + D, D) - (p + 1) * (b.c2 + 1);
-  return b;
-}
-__device__ __forceinline__ int bs_row_base(const BsRows& b, int p, int r) {
-  return r <= p ? b.o1 + r * b.c1 : b.o2 + r * b.c2;
-}
-
-// all loads are issued before the barrier that follows, so the stage pays one
-// global-memory latency for its matrices, not one per gate
+// All gate regions (LDS [0, total), laid out by the planner) in batches of
+// kStageBatch elements per thread: a uniform loop over the gates finds each
+// element's gate, then the batch's table loads are issued together, so the
+// stage pays about one memory latency for its matrices rather than one per
+// gate.  Alignment slots between regions are zero.
+constexpr int kStageBatch = 4;
 template <typename T, int D, int NT>
 __device__ __forceinline__ void stage_all_matrices(const C<T>* __restrict__ table, const sfamd::StageArgs& a,
                                                    bool derivs, C<T>* mats) {
-  for (int gi = 0; gi < a.ngates; ++gi) {
-    const sfamd::StageGate& g = a.g[gi];
-    const C<T>* src = table + g.off;
-    C<T>* dst = mats + g.moff;
-    if (D <= kFullBsMaxD && g.kind == SFAMD_OP_BEAMSPLITTER) {
-      constexpr int D3 = D * D * D;
-      const int nb = (int)bs_nb(D);
-      for (int i = threadIdx.x; i < (derivs ? 2 : 1) * D3; i += NT) {
-        const int part = i / D3, e = i - part * D3;
-        const int r = e / (D * D), k = (e / D) % D, p = e % D;
-        C<T> v = mk<T>(0, 0);
-        if ((r <= p) == (k <= p)) v = src[part * nb + bs_row_base(bs_rows<D>(p), p, r) + k];
-        dst[i] = v;
+  const int total = derivs ? a.mat_adj : a.mat_fwd;
+  for (int base = threadIdx.x; base < total; base += kStageBatch * NT) {
+    int64_t src[kStageBatch];
+#pragma unroll
+    for (int u = 0; u < kStageBatch; ++u) src[u] = -1;
+    for (int g = 0; g < a.ngates; ++g) {  // uniform
+      const int m0 = derivs ? a.g[g].amoff : a.g[g].moff;
+      const int cnt = gate_table_elems<D>(a.g[g].kind, derivs);
+      const int64_t o = a.g[g].off;
+#pragma unroll
+      for (int u = 0; u < kStageBatch; ++u) {
+        const int l = base + u * NT - m0;
+        if (l >= 0 && l < cnt) src[u] = o + l;
       }
-    } else {
-      const int cnt = gate_table_elems<D>(g.kind, derivs);
-      for (int i = threadIdx.x; i < cnt + D; i += NT) dst[i] = i < cnt ? src[i] : mk<T>(0, 0);
     }
+    C<T> v[kStageBatch];
+#pragma unroll
+    for (int u = 0; u < kStageBatch; ++u) v[u] = src[u] >= 0 ? table[src[u]] : mk<T>(0, 0);
+#pragma unroll
+    for (int u = 0; u < kStageBatch; ++u)
+      if (base + u * NT < total) mats[base + u * NT] = v[u];
   }
 }
 
@@ -195,322 +219,132 @@ __device__ __forceinline__ T im_conj_mul(C<T> a, C<T> b) {
   return a.re * b.im - a.im * b.re;
 }
 
-// ---- fiber geometry with compile-time axes ----------------------------------
-// axis A of a tile of K local modes: tile stride D^(K-1-A) and the stride in
-// the padded LDS tile
-template <int D, int K, int A>
-struct Ax {
-  static constexpr int S = Pow<D, K - 1 - A>::v;
-  static constexpr int PS = S == 1 ? 1 : S + S / D;
-};
-
-// tile element with digit 0 on axes A0 and A1 (A1 = A0 for one axis), the
-// other digits taken from f (slowest first)
-template <int D, int K, int A0, int A1>
-__device__ __forceinline__ int fiber_base(int f) {
-  int e = 0, rem = f, mul = 1;
-#pragma unroll
-  for (int t = K - 1; t >= 0; --t) {
-    if (t != A0 && t != A1) {
-      const int q = rem / D;
-      e += (rem - q * D) * mul;
-      rem = q;
-    }
-    mul *= D;
-  }
-  return e;
-}
-
-// beamsplitter fiber (z, p) on axes (A0, A1): element r has first-axis digit r
-// and second-axis digit (p - r) mod D
-template <int D>
-__device__ __forceinline__ int bs_q(int p, int r) {
-  return r <= p ? p - r : p + D - r;
-}
-
-// ---- forward of one gate on a tile ------------------------------------------
-// `put(P, goff, v)` stores one output element: into the next LDS tile at
-// padded position P, or straight to global memory at tile offset goff for the
-// last gate of a forward stage (goff is computed only when the store uses it).
-template <typename T, int D, int K, int A, typename Put>
-__device__ __forceinline__ void fwd_single(int kind, const C<T>* __restrict__ mat, const C<T>* __restrict__ src,
-                                           const sfamd::StageArgs& a, Put put) {
+// ---- forward of one gate: the thread's amplitudes -----------------------------
+// `put(j, e, v)` stores the output amplitude of the thread's j-th element e:
+// into the next LDS tile, or straight to global memory for the last gate of a
+// forward stage.
+template <typename T, int D, int K, typename Put>
+__device__ __forceinline__ void gate_fwd(const sfamd::StageGate& g, const Mine<D, K>& me,
+                                         const C<T>* __restrict__ mat, const C<T>* __restrict__ src, Put put) {
   using Gm = Geo<D, K>;
-  using X = Ax<D, K, A>;
-  constexpr int R = Gm::R;
-  constexpr bool full_rows = Gm::G * R == D;
-  for (int it = threadIdx.x; it < Gm::ITEMS; it += Gm::NT) {
-    const int f = it % Gm::F;
-    const int r0 = (it / Gm::F) * R;
-    const int e0 = fiber_base<D, K, A, A>(f);
-    const int P0 = pad<D>(e0);
-    auto go = [&](int r) { return local_offset<D, K>(e0, a) + (int64_t)r * a.loc_stride[A]; };
+  const int kind = g.kind, a0 = g.a0;
+  const int s0 = axis_stride<D, K>(a0);
+#pragma unroll
+  for (int j = 0; j < Gm::PER; ++j) {
+    if (!owns<D, K>(j)) continue;
+    const int e = threadIdx.x + j * Gm::NT;
+    const int i = pick(me.dg[j], a0);
+    C<T> acc = mk<T>(0, 0);
     if (kind == SFAMD_OP_ROTATION || kind == SFAMD_OP_KERR) {
+      acc = cmul(mat[i], src[e]);
+    } else if (kind == SFAMD_OP_BEAMSPLITTER) {
+      // block s = i + m: out (i, m) <- in (k, s - k), k in [lo, hi]
+      const int a1 = g.a1, s1 = axis_stride<D, K>(a1);
+      const int m = pick(me.dg[j], a1);
+      const int s = i + m;
+      const int lo = bs_lo(s, D), hi = bs_hi(s, D), c = hi - lo + 1;
+      const C<T>* row = mat + bs_block_off(s, D) + (i - lo) * c - lo;  // B_s[i][k] at row[k]
+      // input (k, s - k) sits at e + (k - i) (s0 - s1)
+      const int ds = s0 - s1;
+      const int bk = e - i * ds;
 #pragma unroll
-      for (int j = 0; j < R; ++j) {
-        const int r = r0 + j;
-        if (full_rows || r < D) put(P0 + r * X::PS, go(r), cmul(mat[r], src[P0 + r * X::PS]));
+      for (int k = 0; k < D; ++k)
+        if (k >= lo && k <= hi) cfma(acc, row[k], src[bk + k * ds]);
+    } else if (kind == SFAMD_OP_SQUEEZE) {  // zero unless i + k even
+      const int base = e - i * s0;
+      const C<T>* row = mat + i * D;
+      const int par = i & 1;
+#pragma unroll
+      for (int h = 0; h < (D + 1) / 2; ++h) {
+        const int k = 2 * h + par;
+        if (k < D) cfma(acc, row[k], src[base + k * s0]);
       }
-      continue;
+    } else {  // displacement: dense row
+      const int base = e - i * s0;
+      const C<T>* row = mat + i * D;
+#pragma unroll
+      for (int k = 0; k < D; ++k) cfma(acc, row[k], src[base + k * s0]);
     }
-    C<T> x[D];
-#pragma unroll
-    for (int k = 0; k < D; ++k) x[k] = src[P0 + k * X::PS];
-    const bool sq = kind == SFAMD_OP_SQUEEZE;
-#pragma unroll
-    for (int j = 0; j < R; ++j) {
-      const int r = r0 + j;
-      if (!full_rows && r >= D) continue;
-      const C<T>* row = mat + r * D;
-      C<T> acc = mk<T>(0, 0);
-      if (sq) {  // zero unless r + k even; r0 is even
-#pragma unroll
-        for (int k = j & 1; k < D; k += 2) cfma(acc, row[k], x[k]);
-      } else {
-#pragma unroll
-        for (int k = 0; k < D; ++k) cfma(acc, row[k], x[k]);
-      }
-      put(P0 + r * X::PS, go(r), acc);
-    }
+    put(j, e, acc);
   }
 }
 
-template <typename T, int D, int K, int A0, int A1, typename Put>
-__device__ __forceinline__ void fwd_bs(const C<T>* __restrict__ mat, const C<T>* __restrict__ src,
-                                       const sfamd::StageArgs& a, Put put) {
+// ---- adjoint of one gate: the thread's amplitudes as input indices -----------
+// `put(j, e, v)` stores lam_in: into the other LDS cotangent tile, or straight
+// to global memory for the stage's first gate.
+template <typename T, int D, int K, typename Put>
+__device__ __forceinline__ void gate_adj(const sfamd::StageGate& g, const Mine<D, K>& me,
+                                         const C<T>* __restrict__ mat, const C<T>* __restrict__ pin,
+                                         const C<T>* __restrict__ pout, const C<T>* __restrict__ lo_, Put put,
+                                         double& g0, double& g1) {
   using Gm = Geo<D, K>;
-  constexpr int R = Gm::R;
-  constexpr bool full_rows = Gm::G * R == D;
-  constexpr int PS0 = Ax<D, K, A0>::PS, PS1 = Ax<D, K, A1>::PS;
-  for (int it = threadIdx.x; it < Gm::ITEMS; it += Gm::NT) {
-    const int f = it % Gm::F;
-    const int r0 = (it / Gm::F) * R;
-    const int p = f % D;
-    const int e0 = fiber_base<D, K, A0, A1>(f / D);
-    const int Pp = pad<D>(e0) + p * PS1;  // element r at Pp + r (PS0 - PS1) (+ D PS1 when r > p)
-    auto pos = [&](int r) { return Pp + r * (PS0 - PS1) + (r > p ? D * PS1 : 0); };
-    auto go = [&](int r) {
-      return local_offset<D, K>(e0, a) + (int64_t)r * a.loc_stride[A0] + (int64_t)bs_q<D>(p, r) * a.loc_stride[A1];
-    };
-    C<T> x[D];
-#pragma unroll
-    for (int k = 0; k < D; ++k) x[k] = src[pos(k)];
-    if constexpr (D <= kFullBsMaxD) {
-      const C<T>* mp = mat + p;  // M[r][k][p] at (r D + k) D + p
-#pragma unroll
-      for (int j = 0; j < R; ++j) {
-        const int r = r0 + j;
-        if (!full_rows && r >= D) continue;
-        const C<T>* row = mp + r * D * D;
-        C<T> acc = mk<T>(0, 0);
-#pragma unroll
-        for (int k = 0; k < D; ++k) cfma(acc, row[k * D], x[k]);
-        put(pos(r), go(r), acc);
-      }
-    } else {
-      const BsRows br = bs_rows<D>(p);
-#pragma unroll
-      for (int j = 0; j < R; ++j) {
-        const int r = r0 + j;
-        if (!full_rows && r >= D) continue;
-        const bool up = r <= p;
-        const C<T>* row = mat + bs_row_base(br, p, r);
-        C<T> acc = mk<T>(0, 0);
-#pragma unroll
-        for (int k = 0; k < D; ++k) cfma(acc, row[k], ((k <= p) == up) ? x[k] : mk<T>(0, 0));
-        put(pos(r), go(r), acc);
-      }
-    }
-  }
-}
-
-// ---- adjoint of one gate on a tile -------------------------------------------
-// Item rows are INPUT indices k of the gate: lam_in[k] = sum_i conj(G[i][k])
-// lam_out[i] over the fiber's lam_out (in registers), and the parameter sums
-// of the header over the item's elements (the items' rows partition the tile).
-// `put(P, goff, v)` stores lam_in: into the other LDS cotangent tile, or
-// straight to global memory for the stage's first gate.
-template <typename T, int D, int K, int A, typename Put>
-__device__ __forceinline__ void adj_single(int kind, const C<T>* __restrict__ mat, const C<T>* __restrict__ pin,
-                                           const C<T>* __restrict__ pout, const C<T>* __restrict__ lo,
-                                           const sfamd::StageArgs& a, Put put, double& g0, double& g1) {
-  using Gm = Geo<D, K>;
-  using X = Ax<D, K, A>;
-  constexpr int R = Gm::R;
-  constexpr bool full_rows = Gm::G * R == D;
   constexpr int DD = D * D;
-  for (int it = threadIdx.x; it < Gm::ITEMS; it += Gm::NT) {
-    const int f = it % Gm::F;
-    const int r0 = (it / Gm::F) * R;
-    const int e0 = fiber_base<D, K, A, A>(f);
-    const int P0 = pad<D>(e0);
-    auto go = [&](int r) { return local_offset<D, K>(e0, a) + (int64_t)r * a.loc_stride[A]; };
+  const int kind = g.kind, a0 = g.a0;
+  const int s0 = axis_stride<D, K>(a0);
+#pragma unroll
+  for (int j = 0; j < Gm::PER; ++j) {
+    if (!owns<D, K>(j)) continue;
+    const int e = threadIdx.x + j * Gm::NT;
+    const int k = pick(me.dg[j], a0);
+    const C<T> lk = lo_[e], xo = pout[e], xi = pin[e];
+    C<T> acc = mk<T>(0, 0), mu = mk<T>(0, 0);
+    T f = (T)0;
     if (kind == SFAMD_OP_ROTATION || kind == SFAMD_OP_KERR) {
-      const bool kerr = kind == SFAMD_OP_KERR;
+      acc = cmul(cconj(mat[k]), lk);
+    } else if (kind == SFAMD_OP_BEAMSPLITTER) {
+      // input (k, m) of block s = k + m feeds outputs (N, s - N), N in [lo, hi]
+      const int a1 = g.a1, s1 = axis_stride<D, K>(a1);
+      const int m = pick(me.dg[j], a1);
+      const int s = k + m;
+      const int lo = bs_lo(s, D), hi = bs_hi(s, D), c = hi - lo + 1;
+      const int nb = (int)bs_nb(D);
+      const C<T>* col = mat + bs_block_off(s, D) + (k - lo) - lo * c;  // B[N][k] at col[N c]
+      // output (N, s - N) sits at e + (N - k) (s0 - s1)
+      const int ds = s0 - s1;
+      const int bk = e - k * ds;
 #pragma unroll
-      for (int j = 0; j < R; ++j) {
-        const int k = r0 + j;
-        if (!full_rows && k >= D) continue;
-        const int P = P0 + k * X::PS;
-        const C<T> l = lo[P], fo = pout[P];
-        put(P, go(k), cmul(cconj(mat[k]), l));
-        const T w = (T)(kerr ? k * k : k);
-        g0 += (double)(-w * im_conj_mul(l, fo));  // Re conj(l) i w psi_out
+      for (int N = 0; N < D; ++N) {
+        if (N >= lo && N <= hi) {
+          const C<T> l = lo_[bk + N * ds];
+          cfma_conj(acc, col[N * c], l);
+          cfma_conj(mu, col[nb + N * c], l);
+        }
       }
-      continue;
-    }
-    C<T> l[D];
-#pragma unroll
-    for (int i = 0; i < D; ++i) l[i] = lo[P0 + i * X::PS];
-    const bool sq = kind == SFAMD_OP_SQUEEZE;
-    const T fph = sq ? (T)0.5 : (T)1;
-#pragma unroll
-    for (int j = 0; j < R; ++j) {
-      const int k = r0 + j;
-      if (!full_rows && k >= D) continue;
-      C<T> acc = mk<T>(0, 0), mu = mk<T>(0, 0);
+      f = (T)-1;  // dB/dphi = i (n - N) B: the opposite sign of D and S
+    } else {
+      const int base = e - k * s0;
       const C<T>* col = mat + k;
-      if (sq) {  // i + k even; r0 is even
+      if (kind == SFAMD_OP_SQUEEZE) {  // i + k even
+        const int par = k & 1;
 #pragma unroll
-        for (int i = j & 1; i < D; i += 2) {
-          cfma_conj(acc, col[i * D], l[i]);
-          cfma_conj(mu, col[DD + i * D], l[i]);
+        for (int h = 0; h < (D + 1) / 2; ++h) {
+          const int i = 2 * h + par;
+          if (i < D) {
+            const C<T> l = lo_[base + i * s0];
+            cfma_conj(acc, col[i * D], l);
+            cfma_conj(mu, col[DD + i * D], l);
+          }
         }
+        f = (T)0.5;
       } else {
 #pragma unroll
         for (int i = 0; i < D; ++i) {
-          cfma_conj(acc, col[i * D], l[i]);
-          cfma_conj(mu, col[DD + i * D], l[i]);
+          const C<T> l = lo_[base + i * s0];
+          cfma_conj(acc, col[i * D], l);
+          cfma_conj(mu, col[DD + i * D], l);
         }
+        f = (T)1;
       }
-      const int P = P0 + k * X::PS;
-      put(P, go(k), acc);
-      const C<T> xi = pin[P], xo = pout[P], lk = lo[P];
-      g0 += (double)re_conj_mul(mu, xi);
-      // dG/dphi = i f (m - n) G with m, n the mode's out / in digits
-      g1 += (double)(fph * (T)k * (im_conj_mul(acc, xi) - im_conj_mul(lk, xo)));
     }
+    put(j, e, acc);
+    // R, K: Re conj(lam) i w psi_out with w = k, k^2 (one parameter)
+    const bool diag = kind == SFAMD_OP_ROTATION || kind == SFAMD_OP_KERR;
+    const T w = (T)(kind == SFAMD_OP_KERR ? k * k : k);
+    const T d0 = diag ? -w * im_conj_mul(lk, xo) : re_conj_mul(mu, xi);
+    const T d1 = f * (T)k * (im_conj_mul(acc, xi) - im_conj_mul(lk, xo));
+    g0 += (double)d0;
+    g1 += (double)d1;
   }
-}
-
-template <typename T, int D, int K, int A0, int A1, typename Put>
-__device__ __forceinline__ void adj_bs(const C<T>* __restrict__ mat, const C<T>* __restrict__ pin,
-                                       const C<T>* __restrict__ pout, const C<T>* __restrict__ lo,
-                                       const sfamd::StageArgs& a, Put put, double& g0, double& g1) {
-  using Gm = Geo<D, K>;
-  constexpr int R = Gm::R;
-  constexpr bool full_rows = Gm::G * R == D;
-  constexpr int PS0 = Ax<D, K, A0>::PS, PS1 = Ax<D, K, A1>::PS;
-  for (int it = threadIdx.x; it < Gm::ITEMS; it += Gm::NT) {
-    const int f = it % Gm::F;
-    const int r0 = (it / Gm::F) * R;
-    const int p = f % D;
-    const int e0 = fiber_base<D, K, A0, A1>(f / D);
-    const int Pp = pad<D>(e0) + p * PS1;
-    auto pos = [&](int r) { return Pp + r * (PS0 - PS1) + (r > p ? D * PS1 : 0); };
-    auto go = [&](int r) {
-      return local_offset<D, K>(e0, a) + (int64_t)r * a.loc_stride[A0] + (int64_t)bs_q<D>(p, r) * a.loc_stride[A1];
-    };
-    C<T> l[D];
-#pragma unroll
-    for (int i = 0; i < D; ++i) l[i] = lo[pos(i)];
-#pragma unroll
-    for (int j = 0; j < R; ++j) {
-      const int k = r0 + j;
-      if (!full_rows && k >= D) continue;
-      C<T> acc = mk<T>(0, 0), mu = mk<T>(0, 0);
-      if constexpr (D <= kFullBsMaxD) {
-        constexpr int D3 = D * D * D;
-        const C<T>* col = mat + p + k * D;  // M[i][k][p] at (i D + k) D + p
-#pragma unroll
-        for (int i = 0; i < D; ++i) {
-          cfma_conj(acc, col[i * D * D], l[i]);
-          cfma_conj(mu, col[D3 + i * D * D], l[i]);
-        }
-      } else {
-        const BsRows br = bs_rows<D>(p);
-        const int nb = (int)bs_nb(D);
-        const bool up = k <= p;
-#pragma unroll
-        for (int i = 0; i < D; ++i) {
-          const C<T> li = ((i <= p) == up) ? l[i] : mk<T>(0, 0);
-          const int idx = bs_row_base(br, p, i) + k;
-          cfma_conj(acc, mat[idx], li);
-          cfma_conj(mu, mat[nb + idx], li);
-        }
-      }
-      const int P = pos(k);
-      put(P, go(k), acc);
-      const C<T> xi = pin[P], xo = pout[P], lk = lo[P];
-      g0 += (double)re_conj_mul(mu, xi);
-      // dB/dphi = i (n - N) B with n, N the first-mode digits (k here)
-      g1 += (double)((T)k * (im_conj_mul(lk, xo) - im_conj_mul(acc, xi)));
-    }
-  }
-}
-
-// ---- dispatch on the gate's local axes (uniform per workgroup) ----------------
-template <typename T, int D, int K, int A0, typename Put>
-__device__ __forceinline__ void fwd_bs_a1(int a1, const C<T>* mat, const C<T>* src, const sfamd::StageArgs& a,
-                                          Put put) {
-#define SFAMD_BS_A1(X)                                                         \
-  if constexpr (X < K && X != A0) {                                            \
-    if (a1 == X) return fwd_bs<T, D, K, A0, X>(mat, src, a, put);              \
-  }
-  SFAMD_BS_A1(0) SFAMD_BS_A1(1) SFAMD_BS_A1(2) SFAMD_BS_A1(3)
-#undef SFAMD_BS_A1
-}
-
-template <typename T, int D, int K, typename Put>
-__device__ __forceinline__ void gate_fwd(const sfamd::StageGate& g, const C<T>* __restrict__ mat,
-                                         const C<T>* __restrict__ src, const sfamd::StageArgs& a, Put put) {
-  const int kind = g.kind, a0 = g.a0;
-  if (kind == SFAMD_OP_BEAMSPLITTER) {
-#define SFAMD_BS_A0(X) \
-  if constexpr (X < K) \
-    if (a0 == X) return fwd_bs_a1<T, D, K, X>(g.a1, mat, src, a, put);
-    SFAMD_BS_A0(0) SFAMD_BS_A0(1) SFAMD_BS_A0(2) SFAMD_BS_A0(3)
-#undef SFAMD_BS_A0
-    return;
-  }
-#define SFAMD_1_A(X)   \
-  if constexpr (X < K) \
-    if (a0 == X) return fwd_single<T, D, K, X>(kind, mat, src, a, put);
-  SFAMD_1_A(0) SFAMD_1_A(1) SFAMD_1_A(2) SFAMD_1_A(3)
-#undef SFAMD_1_A
-}
-
-template <typename T, int D, int K, int A0, typename Put>
-__device__ __forceinline__ void adj_bs_a1(int a1, const C<T>* mat, const C<T>* pin, const C<T>* pout,
-                                          const C<T>* lo, const sfamd::StageArgs& a, Put put, double& g0,
-                                          double& g1) {
-#define SFAMD_BS_A1(X)                                                              \
-  if constexpr (X < K && X != A0) {                                                 \
-    if (a1 == X) return adj_bs<T, D, K, A0, X>(mat, pin, pout, lo, a, put, g0, g1); \
-  }
-  SFAMD_BS_A1(0) SFAMD_BS_A1(1) SFAMD_BS_A1(2) SFAMD_BS_A1(3)
-#undef SFAMD_BS_A1
-}
-
-template <typename T, int D, int K, typename Put>
-__device__ __forceinline__ void gate_adj(const sfamd::StageGate& g, const C<T>* __restrict__ mat,
-                                         const C<T>* __restrict__ pin, const C<T>* __restrict__ pout,
-                                         const C<T>* __restrict__ lo, const sfamd::StageArgs& a, Put put, double& g0,
-                                         double& g1) {
-  const int kind = g.kind, a0 = g.a0;
-  if (kind == SFAMD_OP_BEAMSPLITTER) {
-#define SFAMD_BS_A0(X) \
-  if constexpr (X < K) \
-    if (a0 == X) return adj_bs_a1<T, D, K, X>(g.a1, mat, pin, pout, lo, a, put, g0, g1);
-    SFAMD_BS_A0(0) SFAMD_BS_A0(1) SFAMD_BS_A0(2) SFAMD_BS_A0(3)
-#undef SFAMD_BS_A0
-    return;
-  }
-#define SFAMD_1_A(X)   \
-  if constexpr (X < K) \
-    if (a0 == X) return adj_single<T, D, K, X>(kind, mat, pin, pout, lo, a, put, g0, g1);
-  SFAMD_1_A(0) SFAMD_1_A(1) SFAMD_1_A(2) SFAMD_1_A(3)
-#undef SFAMD_1_A
 }
 
 // One DPP lane move of a double (two 32-bit moves); lanes of rows outside
@@ -561,56 +395,48 @@ __device__ __forceinline__ TileRange xcd_tiles(int64_t ntiles) {
 
 // ---- kernels ------------------------------------------------------------
 template <typename T, int D, int K>
-__global__ void __launch_bounds__(sfamd::stage_threads(D, K), 2) k_stage_fwd(const C<T>* __restrict__ in, C<T>* __restrict__ out,
+__global__ void __launch_bounds__(sfamd::stage_threads(D, K)) k_stage_fwd(const C<T>* __restrict__ in, C<T>* __restrict__ out,
                                                              const C<T>* __restrict__ table, sfamd::StageArgs a) {
   using Gm = Geo<D, K>;
   extern __shared__ __align__(16) unsigned char smem[];
   C<T>* bufA = reinterpret_cast<C<T>*>(smem);
-  C<T>* bufB = bufA + Gm::PDK;
-  C<T>* mats = bufB + Gm::PDK;
-  constexpr int PER = (Gm::DK + Gm::NT - 1) / Gm::NT;  // tile elements per thread
+  C<T>* bufB = bufA + Gm::DK;
+  C<T>* mats = bufB + Gm::DK;
   const TileRange tr = xcd_tiles(a.ntiles);
+  const Mine<D, K> me(a);
   // the first tile's loads are in flight while the matrices are staged, and
   // each next tile's while the gates of the current one run
-  C<T> pre[PER];
-  auto fetch = [&](int64_t t) {
-    const int64_t b = tile_base(t, a, D);
+  C<T> pre[Gm::PER];
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int e = threadIdx.x + i * Gm::NT;
-      if (e < Gm::DK) pre[i] = in[b + local_offset<D, K>(e, a)];
-    }
-  };
-  if (tr.first < tr.end) fetch(tr.first);
+  for (int j = 0; j < Gm::PER; ++j) pre[j] = mk<T>(0, 0);
+  if (tr.first < tr.end) fetch_tile<T, D, K>(in, tile_base(tr.first, a, D), me, pre);
   stage_all_matrices<T, D, Gm::NT>(table, a, false, mats);
   for (int64_t t = tr.first; t < tr.end; t += tr.step) {
     const int64_t base = tile_base(t, a, D);
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int e = threadIdx.x + i * Gm::NT;
-      if (e < Gm::DK) bufA[pad<D>(e)] = pre[i];
-    }
+    for (int j = 0; j < Gm::PER; ++j)
+      if (owns<D, K>(j)) bufA[threadIdx.x + j * Gm::NT] = pre[j];
     __syncthreads();
-    if (t + tr.step < tr.end) fetch(t + tr.step);
+    if (t + tr.step < tr.end) fetch_tile<T, D, K>(in, tile_base(t + tr.step, a, D), me, pre);
     C<T>* src = bufA;
     C<T>* dst = bufB;
     for (int gi = 0; gi < a.ngates - 1; ++gi) {
-      gate_fwd<T, D, K>(a.g[gi], mats + a.g[gi].moff, src, a, [dst](int P, int64_t, C<T> v) { dst[P] = v; });
+      const sfamd::StageGate& g = a.g[gi];
+      gate_fwd<T, D, K>(g, me, mats + g.moff, src, [dst](int, int e, C<T> v) { dst[e] = v; });
       __syncthreads();
       C<T>* tmp = src;
       src = dst;
       dst = tmp;
     }
     // the last gate writes its outputs straight to the state
-    const int gl = a.ngates - 1;
-    gate_fwd<T, D, K>(a.g[gl], mats + a.g[gl].moff, src, a,
-                      [&](int, int64_t go, C<T> v) { out[base + go] = v; });
+    const sfamd::StageGate& gl = a.g[a.ngates - 1];
+    gate_fwd<T, D, K>(gl, me, mats + gl.moff, src, [&](int j, int, C<T> v) { out[base + me.goff[j]] = v; });
     __syncthreads();
   }
 }
 
 template <typename T, int D, int K>
-__global__ void __launch_bounds__(sfamd::stage_threads(D, K), 2) k_stage_adj(const C<T>* __restrict__ psi_in,
+__global__ void __launch_bounds__(sfamd::stage_threads(D, K)) k_stage_adj(const C<T>* __restrict__ psi_in,
                                                              const C<T>* __restrict__ lam_out,
                                                              C<T>* __restrict__ lam_in,
                                                              const C<T>* __restrict__ table, sfamd::StageArgs a,
@@ -620,65 +446,62 @@ __global__ void __launch_bounds__(sfamd::stage_threads(D, K), 2) k_stage_adj(con
   // layout: [ngates + 1 psi tiles (every gate's input, then the last output)]
   //         [2 lam tiles][matrices + derivatives][grad slots [ngates][2][waves]]
   C<T>* tiles = reinterpret_cast<C<T>*>(smem);
-  C<T>* lamA = tiles + (a.ngates + 1) * Gm::PDK;
-  C<T>* lamB = lamA + Gm::PDK;
-  C<T>* mats = lamB + Gm::PDK;
+  C<T>* lamA = tiles + (a.ngates + 1) * Gm::DK;
+  C<T>* lamB = lamA + Gm::DK;
+  C<T>* mats = lamB + Gm::DK;
   double* gsl = reinterpret_cast<double*>(mats + a.mat_adj);
   constexpr int kWaves = Gm::NT / 64;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  constexpr int PER = (Gm::DK + Gm::NT - 1) / Gm::NT;  // tile elements per thread
   const TileRange tr = xcd_tiles(a.ntiles);
+  const Mine<D, K> me(a);
   // tile loads in flight while the matrices are staged (first tile) and while
   // the current tile's gates run (the next one)
-  C<T> pre_psi[PER], pre_lam[PER];
-  auto fetch = [&](int64_t t) {
-    const int64_t b = tile_base(t, a, D);
+  C<T> pre_psi[Gm::PER], pre_lam[Gm::PER];
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int e = threadIdx.x + i * Gm::NT;
-      if (e < Gm::DK) {
-        const int64_t off = b + local_offset<D, K>(e, a);
-        pre_psi[i] = psi_in[off];
-        pre_lam[i] = lam_out[off];
-      }
-    }
-  };
-  if (tr.first < tr.end) fetch(tr.first);
+  for (int j = 0; j < Gm::PER; ++j) pre_psi[j] = pre_lam[j] = mk<T>(0, 0);
+  if (tr.first < tr.end) {
+    const int64_t b = tile_base(tr.first, a, D);
+    fetch_tile<T, D, K>(psi_in, b, me, pre_psi);
+    fetch_tile<T, D, K>(lam_out, b, me, pre_lam);
+  }
   for (int idx = threadIdx.x; idx < a.ngates * 2 * kWaves; idx += Gm::NT) gsl[idx] = 0.0;
   stage_all_matrices<T, D, Gm::NT>(table, a, true, mats);
   for (int64_t t = tr.first; t < tr.end; t += tr.step) {
     const int64_t base = tile_base(t, a, D);
 #pragma unroll
-    for (int i = 0; i < PER; ++i) {
-      const int e = threadIdx.x + i * Gm::NT;
-      if (e < Gm::DK) {
-        tiles[pad<D>(e)] = pre_psi[i];
-        lamA[pad<D>(e)] = pre_lam[i];
+    for (int j = 0; j < Gm::PER; ++j) {
+      if (owns<D, K>(j)) {
+        tiles[threadIdx.x + j * Gm::NT] = pre_psi[j];
+        lamA[threadIdx.x + j * Gm::NT] = pre_lam[j];
       }
     }
     __syncthreads();
-    if (t + tr.step < tr.end) fetch(t + tr.step);
+    if (t + tr.step < tr.end) {
+      const int64_t b = tile_base(t + tr.step, a, D);
+      fetch_tile<T, D, K>(psi_in, b, me, pre_psi);
+      fetch_tile<T, D, K>(lam_out, b, me, pre_lam);
+    }
     // recompute the stage forward, keeping every gate's input and output
     for (int gi = 0; gi < a.ngates; ++gi) {
-      C<T>* dst = tiles + (gi + 1) * Gm::PDK;
-      gate_fwd<T, D, K>(a.g[gi], mats + a.g[gi].moff, tiles + gi * Gm::PDK, a,
-                        [dst](int P, int64_t, C<T> v) { dst[P] = v; });
+      const sfamd::StageGate& g = a.g[gi];
+      C<T>* dst = tiles + (gi + 1) * Gm::DK;
+      gate_fwd<T, D, K>(g, me, mats + g.amoff, tiles + gi * Gm::DK, [dst](int, int e, C<T> v) { dst[e] = v; });
       __syncthreads();
     }
     C<T>* lo_ = lamA;
     C<T>* li = lamB;
     for (int gi = a.ngates - 1; gi >= 0; --gi) {
+      const sfamd::StageGate& g = a.g[gi];
       double g0 = 0.0, g1 = 0.0;
-      const C<T>* pin = tiles + gi * Gm::PDK;
-      const C<T>* pout = pin + Gm::PDK;
+      const C<T>* pin = tiles + gi * Gm::DK;
+      const C<T>* pout = pin + Gm::DK;
       if (gi > 0)
-        gate_adj<T, D, K>(a.g[gi], mats + a.g[gi].moff, pin, pout, lo_, a,
-                          [li](int P, int64_t, C<T> v) { li[P] = v; }, g0, g1);
+        gate_adj<T, D, K>(g, me, mats + g.amoff, pin, pout, lo_, [li](int, int e, C<T> v) { li[e] = v; }, g0, g1);
       else  // the stage's first gate: lam_in goes straight to the output state
-        gate_adj<T, D, K>(a.g[gi], mats + a.g[gi].moff, pin, pout, lo_, a,
-                          [&](int, int64_t go, C<T> v) { lam_in[base + go] = v; }, g0, g1);
+        gate_adj<T, D, K>(g, me, mats + g.amoff, pin, pout, lo_,
+                          [&](int j, int, C<T> v) { lam_in[base + me.goff[j]] = v; }, g0, g1);
       // R and K have one parameter: their second slot stays at its cleared zero
-      const bool two = a.g[gi].kind != SFAMD_OP_ROTATION && a.g[gi].kind != SFAMD_OP_KERR;
+      const bool two = g.kind != SFAMD_OP_ROTATION && g.kind != SFAMD_OP_KERR;
       g0 = wave_sum(g0);
       if (two) g1 = wave_sum(g1);
       if (lane == kWaveSumLane) {  // each wave owns its slot: deterministic, no atomics

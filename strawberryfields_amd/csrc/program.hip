@@ -361,7 +361,7 @@ void absorb(const std::vector<OpPlan>& ops, const std::vector<int>& rem, unsigne
 void plan_run(sfamd_prog* p, const std::vector<int>& run, int Kc) {
   const int n = p->n, D = p->D;
   const int64_t esz = elem_size(p->dtype);
-  const int64_t tile_bytes = stage_tile_pad(D, Kc) * esz;
+  const int64_t tile_bytes = stage_tile(D, Kc) * esz;
   std::vector<int> rem = run;
   std::vector<unsigned> cands;
   // all Kc-subsets when few, else subsets grown from the next ops
@@ -426,7 +426,7 @@ void plan_run(sfamd_prog* p, const std::vector<int>& run, int Kc) {
     a.nnl = nlc;
     a.ntiles = ipow(D, nlc);
     for (int t = 0; t < Kc; ++t) a.loc_stride[t] = ipow(D, n - 1 - lmodes[t]);
-    int64_t ma = 0;
+    int64_t mf = 0, ma = 0;
     for (int gi = 0; gi < a.ngates; ++gi) {
       const OpPlan& op = p->ops[best[gi]];
       StageGate& g = a.g[gi];
@@ -438,12 +438,14 @@ void plan_run(sfamd_prog* p, const std::vector<int>& run, int Kc) {
       }
       g.op = best[gi];
       g.off = op.off;
-      // one region per gate in both kernels (the forward fills its first
-      // part); keep each region 16-byte aligned in LDS
-      g.moff = (int32_t)ma;
+      // one region per gate, 16-byte aligned in LDS: the matrix in the
+      // forward kernel, matrix + derivative in the adjoint
+      g.moff = (int32_t)mf;
+      g.amoff = (int32_t)ma;
+      mf += (stage_gate_elems(op.kind, D, false) + 1) & ~(int64_t)1;
       ma += (stage_gate_elems(op.kind, D, true) + 1) & ~(int64_t)1;
     }
-    a.mat_fwd = (int32_t)ma;
+    a.mat_fwd = (int32_t)mf;
     a.mat_adj = (int32_t)ma;
     p->steps.push_back(st);
     std::vector<int> nrem;
@@ -465,7 +467,7 @@ void plan_steps(sfamd_prog* p) {
   // fusion needs compiled stage kernels and room for at least one gate of any kind
   int gmax = 0;
   if (Kc >= 2 && n <= kStageMaxModes && stage_supported(D, Kc)) {
-    const int64_t one = 4 * stage_tile_pad(D, Kc) * esz + stage_gate_elems(SFAMD_OP_BEAMSPLITTER, D, true) * esz + 256;
+    const int64_t one = 4 * stage_tile(D, Kc) * esz + stage_gate_elems(SFAMD_OP_BEAMSPLITTER, D, true) * esz + 256;
     gmax = one <= kStageLdsBudget ? 1 : 0;
   }
   std::vector<int> run;

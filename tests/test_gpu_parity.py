@@ -433,6 +433,38 @@ def test_fused_stages_match_unfused_and_oracle(dtype, n, D, fuse):
     del flat
 
 
+@pytest.mark.parametrize("D", [9, 11, 12, 13, 15, 16])
+def test_fused_single_gate_adjoint_matches_unfused_at_large_cutoffs(D):
+    """One gate per stage on 3 local modes of 4, every kind and both beamsplitter
+    mode orders, random state and cotangent: the fused stage's lam_in and
+    gradient equal the one-kernel-per-gate adjoint.  At these cutoffs a thread
+    owns 1-4 tile amplitudes (the tile has D^3 > 1024 amplitudes from D = 11),
+    and at D = 12-15 a compiler rewrite of the beamsplitter's fiber index into
+    byte dot products once broke the first amplitude's cotangent reads."""
+    n = 4
+    g = torch.Generator(device="cpu").manual_seed(D)
+    for op in [("BS", (1, 2)), ("BS", (3, 1)), ("S", (2,)), ("D", (1,)), ("R", (3,)), ("K", (2,))]:
+        res = {}
+        for fuse in (0, 3):
+            cc = runtime.CompiledCircuit([op], n, D, False, C64, DEV, fuse=fuse)
+            assert cc.nsteps == 1
+            cc.build_table(torch.tensor([[0.4, 0.3]], dtype=torch.float64, device=DEV))
+            st = torch.randn([D] * n, dtype=C64, generator=g).to(DEV) if fuse == 0 else res[0][3]
+            lam = torch.randn([D] * n, dtype=C64, generator=g).to(DEV) if fuse == 0 else res[0][4]
+            out = cc.new_state(1)
+            cc.forward(0, 1, st, out=out, saved=[st])
+            gin = cc.new_state(0)
+            grads = torch.zeros(2, dtype=torch.float64, device=DEV)
+            cc.adjoint(0, 1, [st], lam, gin, grads)
+            torch.cuda.synchronize()
+            res[fuse] = (to_np(out), to_np(gin), grads.cpu().numpy(), st, lam)
+        assert np.abs(res[0][0] - res[3][0]).max() < 1e-5, op
+        dl = np.abs(res[0][1] - res[3][1]).max()
+        assert np.isfinite(dl) and dl < 1e-5, (op, dl)
+        dg = np.abs(res[0][2] - res[3][2]).max() / max(1.0, np.abs(res[0][2]).max())
+        assert dg < 1e-5, (op, res[0][2], res[3][2])
+
+
 def test_fused_gradient_vs_finite_differences_c128():
     n, D = 4, 5
     rng = np.random.default_rng(31)
