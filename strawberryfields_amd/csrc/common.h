@@ -173,12 +173,16 @@ int diag_run_adj(const void* psi, const void* lam_out, void* lam_in, const void*
 int64_t table_size(int kind, int D);
 
 // one operation of a table build: kind and table offset; the operation's
-// parameters are params[2*(first + i) + slot]
+// parameters are params[2*(first + i) + slot].  An operation inside a fused
+// stage also writes its regions of the stage's matrix images (the LDS layout
+// of the forward and adjoint stage kernels) at img_fwd / img_adj (< 0: none).
 struct BuildDesc {
   int32_t kind;
-  int32_t off;  // complex elements
+  int32_t off;      // complex elements
+  int32_t img_fwd;  // complex elements, or -1
+  int32_t img_adj;  // complex elements, or -1
 };
-constexpr int kMaxBuildPerLaunch = 480;  // kernel-argument space (4 KB)
+constexpr int kMaxBuildPerLaunch = 240;  // kernel-argument space (4 KB)
 struct BuildBatch {
   int32_t first;  // program index of d[0]; < 0: single gate with inline p0, p1
   int32_t pad;

@@ -30,6 +30,8 @@ struct StageArgs {
   int64_t ntiles;     // D^nnl
   int32_t mat_fwd;    // LDS matrix region of the forward kernel (complex elements)
   int32_t mat_adj;    // LDS matrix region of the adjoint kernel (complex elements)
+  int64_t img_fwd;    // gate-table offset of the forward region's image
+  int64_t img_adj;    // gate-table offset of the adjoint region's image
   int64_t loc_stride[4];               // global strides of the local modes
   int64_t nl_stride[kStageMaxModes];   // global strides of the other modes (slowest first)
   StageGate g[kStageMaxGates];
@@ -46,8 +48,12 @@ __host__ __device__ constexpr int stage_threads(int D, int K) {
 }
 
 bool stage_supported(int D, int K);
+// Beamsplitters at cutoffs up to kFullBsMaxD are staged in the zero-padded
+// per-fiber layout (D^3 entries per matrix), larger ones in the compact block
+// layout of the gate table.
+constexpr int kFullBsMaxD = 10;
 // LDS matrix elements of one gate: forward (the matrix) or adjoint (matrix and
-// the first parameter's derivative), in the gate table's compact layout
+// the first parameter's derivative)
 int64_t stage_gate_elems(int kind, int D, bool derivs);
 // dynamic LDS bytes the stage kernels need
 int64_t stage_lds_fwd(int D, int K, int mat_fwd, int dtype);

@@ -19,6 +19,7 @@ int64_t stage_gate_elems(int kind, int D, bool derivs) {
     case SFAMD_OP_SQUEEZE:
       return derivs ? 2 * (int64_t)D * D : (int64_t)D * D;
     case SFAMD_OP_BEAMSPLITTER:
+      if (D <= kFullBsMaxD) return (derivs ? 2 : 1) * ipow(D, 3);
       return derivs ? 2 * bs_nb(D) : bs_nb(D);
     default:
       return 0;

@@ -15,13 +15,15 @@
 // on runtime (workgroup-uniform) axes picks the digits it needs.  A gate's
 // output amplitude reads its fiber from the LDS tile and the matrix row from
 // the LDS matrix region: R and K one term, D a row of D terms, S the terms of
-// the same parity, BS the photon-number block s = n0 + n1 of the compact
-// block table.  One small code path per gate kind keeps the instruction
+// the same parity, BS a row of the zero-padded per-fiber matrix (cutoffs up
+// to kFullBsMaxD: every lane runs the same D terms, no branches) or of the
+// photon-number block s = n0 + n1 of the compact block table.  One small code path per gate kind keeps the instruction
 // footprint small (the stage kernels run once per stage on small states).
 //
-// The stage prologue issues the tile's global loads first, then every gate
-// matrix load of the stage (batched into registers), and only then waits, so
-// a stage pays about one memory latency before its first gate.
+// The stage prologue issues the tile's global loads first, then the loads of
+// the stage's matrix image (one contiguous range of the gate table, written
+// in LDS layout by the table build), and only then waits, so a stage pays
+// about one memory latency before its first gate.
 //
 // The adjoint of a stage loads the stage's input tile (its checkpoint) and the
 // incoming cotangent tile, recomputes the stage forward keeping every gate's
@@ -140,54 +142,18 @@ __device__ __forceinline__ void fetch_tile(const C<T>* __restrict__ src, int64_t
     if (owns<D, K>(j)) v[j] = src[b + me.goff[j]];
 }
 
-// ---- staging of a stage's gate tables into LDS (once per workgroup) -------
-// A gate's LDS region holds the head of its gate-table entry: the matrix
-// (forward) or the matrix and its first parameter's derivative (adjoint), in
-// the table's compact layout (R, K: the diagonal; D, S: row-major D x D; BS:
-// the photon-number blocks).
-template <int D>
-__device__ __forceinline__ int gate_table_elems(int kind, bool derivs) {
-  switch (kind) {
-    case SFAMD_OP_ROTATION:
-    case SFAMD_OP_KERR:
-      return D;
-    case SFAMD_OP_DISPLACEMENT:
-    case SFAMD_OP_SQUEEZE:
-      return derivs ? 2 * D * D : D * D;
-    case SFAMD_OP_BEAMSPLITTER:
-      return derivs ? 2 * (int)bs_nb(D) : (int)bs_nb(D);
-    default:
-      return 0;
-  }
-}
-
-// All gate regions (LDS [0, total), laid out by the planner) in batches of
-// kStageBatch elements per thread: a uniform loop over the gates finds each
-// element's gate, then the batch's table loads are issued together, so the
-// stage pays about one memory latency for its matrices rather than one per
-// gate.  Alignment slots between regions are zero.
+// ---- staging of a stage's matrices into LDS (once per workgroup) ----------
+// The table build wrote the stage's regions, in LDS layout, as one contiguous
+// image (the forward kernel's matrices, or the adjoint's matrices and
+// derivatives): staging is a coalesced copy whose loads are all issued before
+// the first wait, kStageBatch per thread at a time.
 constexpr int kStageBatch = 4;
-template <typename T, int D, int NT>
-__device__ __forceinline__ void stage_all_matrices(const C<T>* __restrict__ table, const sfamd::StageArgs& a,
-                                                   bool derivs, C<T>* mats) {
-  const int total = derivs ? a.mat_adj : a.mat_fwd;
+template <typename T, int NT>
+__device__ __forceinline__ void stage_image(const C<T>* __restrict__ img, int total, C<T>* mats) {
   for (int base = threadIdx.x; base < total; base += kStageBatch * NT) {
-    int64_t src[kStageBatch];
-#pragma unroll
-    for (int u = 0; u < kStageBatch; ++u) src[u] = -1;
-    for (int g = 0; g < a.ngates; ++g) {  // uniform
-      const int m0 = derivs ? a.g[g].amoff : a.g[g].moff;
-      const int cnt = gate_table_elems<D>(a.g[g].kind, derivs);
-      const int64_t o = a.g[g].off;
-#pragma unroll
-      for (int u = 0; u < kStageBatch; ++u) {
-        const int l = base + u * NT - m0;
-        if (l >= 0 && l < cnt) src[u] = o + l;
-      }
-    }
     C<T> v[kStageBatch];
 #pragma unroll
-    for (int u = 0; u < kStageBatch; ++u) v[u] = src[u] >= 0 ? table[src[u]] : mk<T>(0, 0);
+    for (int u = 0; u < kStageBatch; ++u) v[u] = base + u * NT < total ? img[base + u * NT] : mk<T>(0, 0);
 #pragma unroll
     for (int u = 0; u < kStageBatch; ++u)
       if (base + u * NT < total) mats[base + u * NT] = v[u];
@@ -238,18 +204,29 @@ __device__ __forceinline__ void gate_fwd(const sfamd::StageGate& g, const Mine<D
     if (kind == SFAMD_OP_ROTATION || kind == SFAMD_OP_KERR) {
       acc = cmul(mat[i], src[e]);
     } else if (kind == SFAMD_OP_BEAMSPLITTER) {
-      // block s = i + m: out (i, m) <- in (k, s - k), k in [lo, hi]
       const int a1 = g.a1, s1 = axis_stride<D, K>(a1);
       const int m = pick(me.dg[j], a1);
       const int s = i + m;
-      const int lo = bs_lo(s, D), hi = bs_hi(s, D), c = hi - lo + 1;
-      const C<T>* row = mat + bs_block_off(s, D) + (i - lo) * c - lo;  // B_s[i][k] at row[k]
-      // input (k, s - k) sits at e + (k - i) (s0 - s1)
       const int ds = s0 - s1;
-      const int bk = e - i * ds;
+      if constexpr (D <= sfamd::kFullBsMaxD) {
+        // fiber p = s mod D: inputs (k, (p - k) mod D) for every k, weights
+        // M[i][k][p] (zero across the two photon-number blocks); input k sits
+        // at b0 + k (s0 - s1), plus D s1 where the second digit wraps (k > p)
+        const bool upper = s >= D;
+        const int p = upper ? s - D : s, wrap = D * s1;
+        const int b0 = e - i * ds - (upper ? wrap : 0);
+        const C<T>* w = mat + i * D * D + p;  // M[i][k][p] at w[k D]
 #pragma unroll
-      for (int k = 0; k < D; ++k)
-        if (k >= lo && k <= hi) cfma(acc, row[k], src[bk + k * ds]);
+        for (int k = 0; k < D; ++k) cfma(acc, w[k * D], src[b0 + k * ds + (k > p ? wrap : 0)]);
+      } else {
+        // block s = i + m: out (i, m) <- in (k, s - k), k in [lo, hi], at e + (k - i) (s0 - s1)
+        const int lo = bs_lo(s, D), hi = bs_hi(s, D), c = hi - lo + 1;
+        const C<T>* row = mat + bs_block_off(s, D) + (i - lo) * c - lo;  // B_s[i][k] at row[k]
+        const int bk = e - i * ds;
+#pragma unroll
+        for (int k = 0; k < D; ++k)
+          if (k >= lo && k <= hi) cfma(acc, row[k], src[bk + k * ds]);
+      }
     } else if (kind == SFAMD_OP_SQUEEZE) {  // zero unless i + k even
       const int base = e - i * s0;
       const C<T>* row = mat + i * D;
@@ -292,22 +269,38 @@ __device__ __forceinline__ void gate_adj(const sfamd::StageGate& g, const Mine<D
     if (kind == SFAMD_OP_ROTATION || kind == SFAMD_OP_KERR) {
       acc = cmul(cconj(mat[k]), lk);
     } else if (kind == SFAMD_OP_BEAMSPLITTER) {
-      // input (k, m) of block s = k + m feeds outputs (N, s - N), N in [lo, hi]
       const int a1 = g.a1, s1 = axis_stride<D, K>(a1);
       const int m = pick(me.dg[j], a1);
       const int s = k + m;
-      const int lo = bs_lo(s, D), hi = bs_hi(s, D), c = hi - lo + 1;
-      const int nb = (int)bs_nb(D);
-      const C<T>* col = mat + bs_block_off(s, D) + (k - lo) - lo * c;  // B[N][k] at col[N c]
-      // output (N, s - N) sits at e + (N - k) (s0 - s1)
       const int ds = s0 - s1;
-      const int bk = e - k * ds;
+      if constexpr (D <= sfamd::kFullBsMaxD) {
+        // input (k, m) on fiber p = s mod D feeds outputs (N, (p - N) mod D),
+        // weights M[N][k][p], derivatives at D^3 further
+        constexpr int D3 = D * D * D;
+        const bool upper = s >= D;
+        const int p = upper ? s - D : s, wrap = D * s1;
+        const int b0 = e - k * ds - (upper ? wrap : 0);
+        const C<T>* w = mat + k * D + p;  // M[N][k][p] at w[N D^2]
 #pragma unroll
-      for (int N = 0; N < D; ++N) {
-        if (N >= lo && N <= hi) {
-          const C<T> l = lo_[bk + N * ds];
-          cfma_conj(acc, col[N * c], l);
-          cfma_conj(mu, col[nb + N * c], l);
+        for (int N = 0; N < D; ++N) {
+          const C<T> l = lo_[b0 + N * ds + (N > p ? wrap : 0)];
+          cfma_conj(acc, w[N * DD], l);
+          cfma_conj(mu, w[D3 + N * DD], l);
+        }
+      } else {
+        // input (k, m) of block s = k + m feeds outputs (N, s - N), N in [lo, hi],
+        // at e + (N - k) (s0 - s1)
+        const int lo = bs_lo(s, D), hi = bs_hi(s, D), c = hi - lo + 1;
+        const int nb = (int)bs_nb(D);
+        const C<T>* col = mat + bs_block_off(s, D) + (k - lo) - lo * c;  // B[N][k] at col[N c]
+        const int bk = e - k * ds;
+#pragma unroll
+        for (int N = 0; N < D; ++N) {
+          if (N >= lo && N <= hi) {
+            const C<T> l = lo_[bk + N * ds];
+            cfma_conj(acc, col[N * c], l);
+            cfma_conj(mu, col[nb + N * c], l);
+          }
         }
       }
       f = (T)-1;  // dB/dphi = i (n - N) B: the opposite sign of D and S
@@ -410,7 +403,7 @@ __global__ void __launch_bounds__(sfamd::stage_threads(D, K)) k_stage_fwd(const 
 #pragma unroll
   for (int j = 0; j < Gm::PER; ++j) pre[j] = mk<T>(0, 0);
   if (tr.first < tr.end) fetch_tile<T, D, K>(in, tile_base(tr.first, a, D), me, pre);
-  stage_all_matrices<T, D, Gm::NT>(table, a, false, mats);
+  stage_image<T, Gm::NT>(table + a.img_fwd, a.mat_fwd, mats);
   for (int64_t t = tr.first; t < tr.end; t += tr.step) {
     const int64_t base = tile_base(t, a, D);
 #pragma unroll
@@ -465,7 +458,7 @@ __global__ void __launch_bounds__(sfamd::stage_threads(D, K)) k_stage_adj(const 
     fetch_tile<T, D, K>(lam_out, b, me, pre_lam);
   }
   for (int idx = threadIdx.x; idx < a.ngates * 2 * kWaves; idx += Gm::NT) gsl[idx] = 0.0;
-  stage_all_matrices<T, D, Gm::NT>(table, a, true, mats);
+  stage_image<T, Gm::NT>(table + a.img_adj, a.mat_adj, mats);
   for (int64_t t = tr.first; t < tr.end; t += tr.step) {
     const int64_t base = tile_base(t, a, D);
 #pragma unroll

@@ -22,6 +22,7 @@
 // Derivatives are carried through the same recurrences (forward mode), so they
 // are the exact derivatives of the truncated matrices.
 #include "common.h"
+#include "fused.h"
 
 namespace {
 
@@ -523,6 +524,53 @@ __device__ void build_loss(double Tt, int D, cpx<T>* out) {
   }
 }
 
+// One gate's region of a fused stage's matrix image: the head of its table
+// entry (the matrix; for the adjoint also the first parameter's derivative),
+// with beamsplitters up to kFullBsMaxD expanded to the per-fiber layout
+// M[r][k][p] at (r D + k) D + p: row r (output first-mode digit), column k
+// (input first-mode digit) of blockdiag(B_p, B_(p+D)), the gate on the
+// modular anti-diagonal {(k, (p - k) mod D)}; entries joining the two photon-
+// number blocks are zero.
+template <typename T>
+__device__ void write_stage_image(int kind, int D, const cpx<T>* __restrict__ entry, cpx<T>* __restrict__ dst,
+                                  bool derivs) {
+  const int parts = derivs ? 2 : 1;
+  if (kind == SFAMD_OP_BEAMSPLITTER && D <= sfamd::kFullBsMaxD) {
+    const int DD = D * D, D3 = DD * D;
+    const int64_t nb = bs_nb(D);
+    for (int l = threadIdx.x; l < parts * D3; l += blockDim.x) {
+      const int part = l / D3, e = l - part * D3;
+      const int r = e / DD, k = (e / D) % D, p = e % D;
+      cpx<T> v = mk<T>(0, 0);
+      if ((r <= p) == (k <= p)) {
+        const int s = r <= p ? p : p + D;
+        const int lo = bs_lo(s, D), c = bs_hi(s, D) - lo + 1;
+        v = entry[part * nb + bs_block_off(s, D) + (int64_t)(r - lo) * c + (k - lo)];
+      }
+      dst[l] = v;
+    }
+    return;
+  }
+  int64_t cnt;
+  switch (kind) {
+    case SFAMD_OP_ROTATION:
+    case SFAMD_OP_KERR:
+      cnt = D;
+      break;
+    case SFAMD_OP_DISPLACEMENT:
+    case SFAMD_OP_SQUEEZE:
+      cnt = (int64_t)parts * D * D;
+      break;
+    case SFAMD_OP_BEAMSPLITTER:
+      cnt = parts * bs_nb(D);
+      break;
+    default:
+      cnt = 0;
+      break;
+  }
+  for (int64_t l = threadIdx.x; l < cnt; l += blockDim.x) dst[l] = entry[l];
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_build(sfamd::BuildBatch b, int D, const double* params, cpx<T>* table) {
   // one LDS buffer serves every builder: passive-gate blocks, D/S columns, or
@@ -569,6 +617,13 @@ __global__ void __launch_bounds__(256) k_build(sfamd::BuildBatch b, int D, const
       break;
     default:
       break;
+  }
+  if (d.img_fwd >= 0 || d.img_adj >= 0) {
+    // the entry was written by this workgroup: the barrier orders those
+    // stores before the reads below (same compute unit)
+    __syncthreads();
+    if (d.img_fwd >= 0) write_stage_image<T>(d.kind, D, out, table + d.img_fwd, false);
+    if (d.img_adj >= 0) write_stage_image<T>(d.kind, D, out, table + d.img_adj, true);
   }
 }
 

@@ -307,6 +307,7 @@ struct Step {
 struct sfamd_prog {
   int n, D, dtype, mixed_in, nops, nwg, fuse_K;
   std::vector<OpPlan> ops;
+  std::vector<int32_t> img_fwd, img_adj;  // per op: its stage-image regions, or -1
   std::vector<Step> steps;
   std::vector<int64_t> state_elems;  // elements at the input of step i (i = nsteps: output)
   int64_t table_elems, max_state;
@@ -579,6 +580,7 @@ extern "C" int sfamd_build_gate(int kind, double p0, double p1, int D, int dtype
   b.p1 = p1;
   b.d[0].kind = kind;
   b.d[0].off = 0;
+  b.d[0].img_fwd = b.d[0].img_adj = -1;
   return launch_build(b, 1, D, nullptr, out, dtype, (hipStream_t)stream);
 }
 
@@ -682,10 +684,25 @@ extern "C" int sfamd_prog_create(const sfamd_op_t* ops, int nops, int n, int D, 
     // a two-pass op on a DM and a pure->mixed loss need DM-sized scratch
     if (op.mix_first) max_state = ipow(D, 2 * n) > max_state ? ipow(D, 2 * n) : max_state;
   }
-  p->table_elems = off;
   p->max_state = max_state;
   p->final_mixed = mixed;
   plan_steps(p);
+  // the fused stages' matrix images follow the operations' entries
+  p->img_fwd.assign(nops, -1);
+  p->img_adj.assign(nops, -1);
+  for (Step& st : p->steps) {
+    if (!st.fused) continue;
+    StageArgs& a = st.args;
+    a.img_fwd = off;
+    off += a.mat_fwd;
+    a.img_adj = off;
+    off += a.mat_adj;
+    for (int gi = 0; gi < a.ngates; ++gi) {
+      p->img_fwd[a.g[gi].op] = (int32_t)(a.img_fwd + a.g[gi].moff);
+      p->img_adj[a.g[gi].op] = (int32_t)(a.img_adj + a.g[gi].amoff);
+    }
+  }
+  p->table_elems = off;
   p->state_elems.push_back(ipow(D, mixed ? 2 * n : n));
   int64_t nwg = (max_state + kBlock - 1) / kBlock;
   if (nwg > 4096) nwg = 4096;
@@ -735,6 +752,8 @@ extern "C" int sfamd_prog_build_table(const sfamd_prog_t* p, const double* param
     for (int i = 0; i < cnt; ++i) {
       b.d[i].kind = p->ops[first + i].kind;  // MIX entries build nothing
       b.d[i].off = (int32_t)p->ops[first + i].off;
+      b.d[i].img_fwd = p->img_fwd[first + i];
+      b.d[i].img_adj = p->img_adj[first + i];
     }
     RET_IF(launch_build(b, cnt, p->D, params, table, p->dtype, (hipStream_t)stream));
   }
