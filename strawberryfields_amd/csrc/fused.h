@@ -19,7 +19,8 @@ struct StageGate {
   int8_t a1;      // local axis of the second mode (beamsplitter)
   int8_t pad;
   int32_t op;     // program operation index (gradient slot)
-  int64_t off;    // gate-table offset, complex elements
+  int32_t s0;     // tile stride of axis a0 (D^(K-1-a0))
+  int32_t s1;     // tile stride of axis a1
   int32_t moff;   // LDS offset of the matrix, forward layout (matrix only)
   int32_t amoff;  // LDS offset of matrix + derivative, adjoint layout
 };

@@ -66,16 +66,6 @@ struct Geo {
   static constexpr int PER = (DK + NT - 1) / NT;  // tile amplitudes per thread
 };
 
-// tile stride of local axis a (0 = slowest): D^(K-1-a)
-template <int D, int K>
-__device__ __forceinline__ int axis_stride(int a) {
-  int s = 1;
-#pragma unroll
-  for (int t = 0; t < K - 1; ++t)
-    if (t >= a) s *= D;
-  return s;
-}
-
 // digit of the thread's amplitude on the (uniform, runtime) axis a: the K
 // digits are packed 8 bits each (axis t at bit 8 t), so the pick is one
 // bitfield extract with a scalar offset
@@ -95,12 +85,33 @@ __device__ __forceinline__ int64_t local_offset(int e, const sfamd::StageArgs& a
   return off;
 }
 
-__device__ __forceinline__ int64_t tile_base(int64_t t, const sfamd::StageArgs& a, int D) {
+// global offset of tile t: the digits of t (radix D, the last non-local
+// mode fastest) times the non-local strides.  Tile indices fit in 32 bits;
+// the strides of up to 8 non-local modes are loaded before the digit loop
+// (independent scalar loads, one wait).
+template <int D>
+__device__ __forceinline__ int64_t tile_base(int t, const sfamd::StageArgs& a) {
+  constexpr int U = 8;
   int64_t base = 0;
-  for (int u = a.nnl - 1; u >= 0; --u) {
-    const int64_t dg = t % D;
-    t /= D;
-    base += dg * a.nl_stride[u];
+  const int nnl = a.nnl;
+  if (nnl <= U) {
+    int64_t st[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) st[u] = a.nl_stride[u];
+#pragma unroll
+    for (int u = U - 1; u >= 0; --u) {
+      if (u < nnl) {
+        const int q = t / D;
+        base += (int64_t)(t - q * D) * st[u];
+        t = q;
+      }
+    }
+    return base;
+  }
+  for (int u = nnl - 1; u >= 0; --u) {
+    const int q = t / D;
+    base += (int64_t)(t - q * D) * a.nl_stride[u];
+    t = q;
   }
   return base;
 }
@@ -145,18 +156,34 @@ __device__ __forceinline__ void fetch_tile(const C<T>* __restrict__ src, int64_t
 // ---- staging of a stage's matrices into LDS (once per workgroup) ----------
 // The table build wrote the stage's regions, in LDS layout, as one contiguous
 // image (the forward kernel's matrices, or the adjoint's matrices and
-// derivatives): staging is a coalesced copy whose loads are all issued before
-// the first wait, kStageBatch per thread at a time.
-constexpr int kStageBatch = 4;
+// derivatives): staging is a coalesced copy.  The kernels issue the first
+// batch of image loads and the tile loads together and wait once.
+constexpr int kStageBatch = 8;
 template <typename T, int NT>
-__device__ __forceinline__ void stage_image(const C<T>* __restrict__ img, int total, C<T>* mats) {
-  for (int base = threadIdx.x; base < total; base += kStageBatch * NT) {
+__device__ __forceinline__ void image_load(const C<T>* __restrict__ img, int total, int b0, C<T> (&v)[kStageBatch]) {
+  // unconditional loads (index clamped into the image): no branch separates
+  // them, so one wait covers the batch
+#pragma unroll
+  for (int u = 0; u < kStageBatch; ++u) {
+    const int l = b0 + (int)threadIdx.x + u * NT;
+    v[u] = img[l < total ? l : total - 1];
+  }
+}
+template <typename T, int NT>
+__device__ __forceinline__ void image_store(C<T>* mats, int total, int b0, const C<T> (&v)[kStageBatch]) {
+#pragma unroll
+  for (int u = 0; u < kStageBatch; ++u) {
+    const int l = b0 + (int)threadIdx.x + u * NT;
+    if (l < total) mats[l] = v[u];
+  }
+}
+// the image beyond the first batch (large stages only)
+template <typename T, int NT>
+__device__ __forceinline__ void image_rest(const C<T>* __restrict__ img, int total, C<T>* mats) {
+  for (int b0 = kStageBatch * NT; b0 < total; b0 += kStageBatch * NT) {
     C<T> v[kStageBatch];
-#pragma unroll
-    for (int u = 0; u < kStageBatch; ++u) v[u] = base + u * NT < total ? img[base + u * NT] : mk<T>(0, 0);
-#pragma unroll
-    for (int u = 0; u < kStageBatch; ++u)
-      if (base + u * NT < total) mats[base + u * NT] = v[u];
+    image_load<T, NT>(img, total, b0, v);
+    image_store<T, NT>(mats, total, b0, v);
   }
 }
 
@@ -194,7 +221,7 @@ __device__ __forceinline__ void gate_fwd(const sfamd::StageGate& g, const Mine<D
                                          const C<T>* __restrict__ mat, const C<T>* __restrict__ src, Put put) {
   using Gm = Geo<D, K>;
   const int kind = g.kind, a0 = g.a0;
-  const int s0 = axis_stride<D, K>(a0);
+  const int s0 = g.s0;
 #pragma unroll
   for (int j = 0; j < Gm::PER; ++j) {
     if (!owns<D, K>(j)) continue;
@@ -204,7 +231,7 @@ __device__ __forceinline__ void gate_fwd(const sfamd::StageGate& g, const Mine<D
     if (kind == SFAMD_OP_ROTATION || kind == SFAMD_OP_KERR) {
       acc = cmul(mat[i], src[e]);
     } else if (kind == SFAMD_OP_BEAMSPLITTER) {
-      const int a1 = g.a1, s1 = axis_stride<D, K>(a1);
+      const int a1 = g.a1, s1 = g.s1;
       const int m = pick(me.dg[j], a1);
       const int s = i + m;
       const int ds = s0 - s1;
@@ -257,7 +284,7 @@ __device__ __forceinline__ void gate_adj(const sfamd::StageGate& g, const Mine<D
   using Gm = Geo<D, K>;
   constexpr int DD = D * D;
   const int kind = g.kind, a0 = g.a0;
-  const int s0 = axis_stride<D, K>(a0);
+  const int s0 = g.s0;
 #pragma unroll
   for (int j = 0; j < Gm::PER; ++j) {
     if (!owns<D, K>(j)) continue;
@@ -269,7 +296,7 @@ __device__ __forceinline__ void gate_adj(const sfamd::StageGate& g, const Mine<D
     if (kind == SFAMD_OP_ROTATION || kind == SFAMD_OP_KERR) {
       acc = cmul(cconj(mat[k]), lk);
     } else if (kind == SFAMD_OP_BEAMSPLITTER) {
-      const int a1 = g.a1, s1 = axis_stride<D, K>(a1);
+      const int a1 = g.a1, s1 = g.s1;
       const int m = pick(me.dg[j], a1);
       const int s = k + m;
       const int ds = s0 - s1;
@@ -374,15 +401,15 @@ constexpr int kWaveSumLane = 63;
 // contiguous ranges and XCD x's blocks (b = x + 8 k) walk range x; every line
 // is then fetched into one L2 instead of eight.
 struct TileRange {
-  int64_t first, end, step;
+  int first, end, step;
 };
 
-__device__ __forceinline__ TileRange xcd_tiles(int64_t ntiles) {
-  const int64_t G = gridDim.x, b = blockIdx.x;
+__device__ __forceinline__ TileRange xcd_tiles(int ntiles) {
+  const int G = gridDim.x, b = blockIdx.x;
   if (G % 8) return TileRange{b, ntiles, G};
-  const int64_t per = (ntiles + 7) / 8;
-  const int64_t x = b % 8;
-  const int64_t end = (x + 1) * per < ntiles ? (x + 1) * per : ntiles;
+  const int per = (ntiles + 7) / 8;
+  const int x = b % 8;
+  const int end = (x + 1) * per < ntiles ? (x + 1) * per : ntiles;
   return TileRange{x * per + b / 8, end, G / 8};
 }
 
@@ -395,22 +422,26 @@ __global__ void __launch_bounds__(sfamd::stage_threads(D, K)) k_stage_fwd(const 
   C<T>* bufA = reinterpret_cast<C<T>*>(smem);
   C<T>* bufB = bufA + Gm::DK;
   C<T>* mats = bufB + Gm::DK;
-  const TileRange tr = xcd_tiles(a.ntiles);
+  const TileRange tr = xcd_tiles((int)a.ntiles);
   const Mine<D, K> me(a);
   // the first tile's loads are in flight while the matrices are staged, and
   // each next tile's while the gates of the current one run
   C<T> pre[Gm::PER];
 #pragma unroll
   for (int j = 0; j < Gm::PER; ++j) pre[j] = mk<T>(0, 0);
-  if (tr.first < tr.end) fetch_tile<T, D, K>(in, tile_base(tr.first, a, D), me, pre);
-  stage_image<T, Gm::NT>(table + a.img_fwd, a.mat_fwd, mats);
-  for (int64_t t = tr.first; t < tr.end; t += tr.step) {
-    const int64_t base = tile_base(t, a, D);
+  const C<T>* img = table + a.img_fwd;
+  C<T> iv[kStageBatch];
+  image_load<T, Gm::NT>(img, a.mat_fwd, 0, iv);
+  fetch_tile<T, D, K>(in, tile_base<D>(tr.first < tr.end ? tr.first : 0, a), me, pre);
+  image_store<T, Gm::NT>(mats, a.mat_fwd, 0, iv);
+  image_rest<T, Gm::NT>(img, a.mat_fwd, mats);
+  for (int t = tr.first; t < tr.end; t += tr.step) {
+    const int64_t base = tile_base<D>(t, a);
 #pragma unroll
     for (int j = 0; j < Gm::PER; ++j)
       if (owns<D, K>(j)) bufA[threadIdx.x + j * Gm::NT] = pre[j];
     __syncthreads();
-    if (t + tr.step < tr.end) fetch_tile<T, D, K>(in, tile_base(t + tr.step, a, D), me, pre);
+    if (t + tr.step < tr.end) fetch_tile<T, D, K>(in, tile_base<D>(t + tr.step, a), me, pre);
     C<T>* src = bufA;
     C<T>* dst = bufB;
     for (int gi = 0; gi < a.ngates - 1; ++gi) {
@@ -445,22 +476,26 @@ __global__ void __launch_bounds__(sfamd::stage_threads(D, K)) k_stage_adj(const 
   double* gsl = reinterpret_cast<double*>(mats + a.mat_adj);
   constexpr int kWaves = Gm::NT / 64;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const TileRange tr = xcd_tiles(a.ntiles);
+  const TileRange tr = xcd_tiles((int)a.ntiles);
   const Mine<D, K> me(a);
   // tile loads in flight while the matrices are staged (first tile) and while
   // the current tile's gates run (the next one)
   C<T> pre_psi[Gm::PER], pre_lam[Gm::PER];
 #pragma unroll
   for (int j = 0; j < Gm::PER; ++j) pre_psi[j] = pre_lam[j] = mk<T>(0, 0);
-  if (tr.first < tr.end) {
-    const int64_t b = tile_base(tr.first, a, D);
+  const C<T>* img = table + a.img_adj;
+  C<T> iv[kStageBatch];
+  image_load<T, Gm::NT>(img, a.mat_adj, 0, iv);
+  {
+    const int64_t b = tile_base<D>(tr.first < tr.end ? tr.first : 0, a);
     fetch_tile<T, D, K>(psi_in, b, me, pre_psi);
     fetch_tile<T, D, K>(lam_out, b, me, pre_lam);
   }
   for (int idx = threadIdx.x; idx < a.ngates * 2 * kWaves; idx += Gm::NT) gsl[idx] = 0.0;
-  stage_image<T, Gm::NT>(table + a.img_adj, a.mat_adj, mats);
-  for (int64_t t = tr.first; t < tr.end; t += tr.step) {
-    const int64_t base = tile_base(t, a, D);
+  image_store<T, Gm::NT>(mats, a.mat_adj, 0, iv);
+  image_rest<T, Gm::NT>(img, a.mat_adj, mats);
+  for (int t = tr.first; t < tr.end; t += tr.step) {
+    const int64_t base = tile_base<D>(t, a);
 #pragma unroll
     for (int j = 0; j < Gm::PER; ++j) {
       if (owns<D, K>(j)) {
@@ -470,7 +505,7 @@ __global__ void __launch_bounds__(sfamd::stage_threads(D, K)) k_stage_adj(const 
     }
     __syncthreads();
     if (t + tr.step < tr.end) {
-      const int64_t b = tile_base(t + tr.step, a, D);
+      const int64_t b = tile_base<D>(t + tr.step, a);
       fetch_tile<T, D, K>(psi_in, b, me, pre_psi);
       fetch_tile<T, D, K>(lam_out, b, me, pre_lam);
     }

@@ -438,7 +438,8 @@ void plan_run(sfamd_prog* p, const std::vector<int>& run, int Kc) {
         if (op.kind == SFAMD_OP_BEAMSPLITTER && lmodes[t] == op.m1) g.a1 = (int8_t)t;
       }
       g.op = best[gi];
-      g.off = op.off;
+      g.s0 = (int32_t)ipow(D, Kc - 1 - g.a0);
+      g.s1 = (int32_t)ipow(D, Kc - 1 - g.a1);
       // one region per gate, 16-byte aligned in LDS: the matrix in the
       // forward kernel, matrix + derivative in the adjoint
       g.moff = (int32_t)mf;
