@@ -11,7 +11,9 @@ constexpr int kStageMaxTile = 4096;  // amplitudes per LDS tile (D^K)
 constexpr int kStageMaxD = 20;       // cutoffs with compiled stage kernels: 2..20
 constexpr int kStageMaxModes = 24;   // modes outside the tile
 // dynamic LDS a stage may use: gfx950 has 160 KB of LDS per compute unit
+constexpr int64_t kStageLdsPerCu = 160 * 1024;
 constexpr int64_t kStageLdsBudget = 152 * 1024;
+constexpr int64_t kComputeUnits = 256;  // MI355X
 
 struct StageGate {
   int8_t kind;    // SFAMD_OP_ROTATION .. SFAMD_OP_BEAMSPLITTER
@@ -25,10 +27,13 @@ struct StageGate {
   int32_t amoff;  // LDS offset of matrix + derivative, adjoint layout
 };
 
+// The fields a stage kernel's prologue reads come first (one or two scalar
+// cache lines with the pointer arguments), so their loads are issued together.
 struct StageArgs {
   int32_t ngates;
   int32_t nnl;        // modes outside the tile
-  int64_t ntiles;     // D^nnl
+  int32_t ntiles;     // D^nnl
+  int32_t grid;       // workgroups of the launch (set by the launcher)
   int32_t mat_fwd;    // LDS matrix region of the forward kernel (complex elements)
   int32_t mat_adj;    // LDS matrix region of the adjoint kernel (complex elements)
   int64_t img_fwd;    // gate-table offset of the forward region's image

@@ -86,32 +86,26 @@ __device__ __forceinline__ int64_t local_offset(int e, const sfamd::StageArgs& a
 }
 
 // global offset of tile t: the digits of t (radix D, the last non-local
-// mode fastest) times the non-local strides.  Tile indices fit in 32 bits;
-// the strides of up to 8 non-local modes are loaded before the digit loop
-// (independent scalar loads, one wait).
+// mode fastest) times the non-local strides.  Tile indices fit in 32 bits.
+// The kernels load the strides of up to kNlPre non-local modes with the other
+// prologue arguments (one scalar-load round trip); more modes read the rest.
+constexpr int kNlPre = 8;
 template <int D>
-__device__ __forceinline__ int64_t tile_base(int t, const sfamd::StageArgs& a) {
-  constexpr int U = 8;
+__device__ __forceinline__ int64_t tile_base(int t, const sfamd::StageArgs& a, const int64_t (&st)[kNlPre]) {
   int64_t base = 0;
   const int nnl = a.nnl;
-  if (nnl <= U) {
-    int64_t st[U];
-#pragma unroll
-    for (int u = 0; u < U; ++u) st[u] = a.nl_stride[u];
-#pragma unroll
-    for (int u = U - 1; u >= 0; --u) {
-      if (u < nnl) {
-        const int q = t / D;
-        base += (int64_t)(t - q * D) * st[u];
-        t = q;
-      }
-    }
-    return base;
-  }
-  for (int u = nnl - 1; u >= 0; --u) {
+  for (int u = nnl - 1; u >= kNlPre; --u) {  // (rarely) modes beyond the preloaded strides: the fastest
     const int q = t / D;
     base += (int64_t)(t - q * D) * a.nl_stride[u];
     t = q;
+  }
+#pragma unroll
+  for (int u = kNlPre - 1; u >= 0; --u) {
+    if (u < nnl) {
+      const int q = t / D;
+      base += (int64_t)(t - q * D) * st[u];
+      t = q;
+    }
   }
   return base;
 }
@@ -280,7 +274,7 @@ template <typename T, int D, int K, typename Put>
 __device__ __forceinline__ void gate_adj(const sfamd::StageGate& g, const Mine<D, K>& me,
                                          const C<T>* __restrict__ mat, const C<T>* __restrict__ pin,
                                          const C<T>* __restrict__ pout, const C<T>* __restrict__ lo_, Put put,
-                                         double& g0, double& g1) {
+                                         T& g0, T& g1) {
   using Gm = Geo<D, K>;
   constexpr int DD = D * D;
   const int kind = g.kind, a0 = g.a0;
@@ -362,8 +356,8 @@ __device__ __forceinline__ void gate_adj(const sfamd::StageGate& g, const Mine<D
     const T w = (T)(kind == SFAMD_OP_KERR ? k * k : k);
     const T d0 = diag ? -w * im_conj_mul(lk, xo) : re_conj_mul(mu, xi);
     const T d1 = f * (T)k * (im_conj_mul(acc, xi) - im_conj_mul(lk, xo));
-    g0 += (double)d0;
-    g1 += (double)d1;
+    g0 += d0;
+    g1 += d1;
   }
 }
 
@@ -395,6 +389,23 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 constexpr int kWaveSumLane = 63;
 
+// the same reduction in single precision (complex64 stages: 64 lanes' partial
+// products, well inside single precision; the per-wave sums are added in
+// double), one DPP add per step
+template <int Ctrl, int RowMask>
+__device__ __forceinline__ float dpp_move_f(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), Ctrl, RowMask, 0xf, false));
+}
+__device__ __forceinline__ float wave_sum(float v) {
+  v += dpp_move_f<0xB1, 0xF>(v);
+  v += dpp_move_f<0x4E, 0xF>(v);
+  v += dpp_move_f<0x141, 0xF>(v);
+  v += dpp_move_f<0x140, 0xF>(v);
+  v += dpp_move_f<0x142, 0xA>(v);
+  v += dpp_move_f<0x143, 0xC>(v);
+  return v;
+}
+
 // Workgroups are dealt to the 8 XCDs round-robin (block b runs on XCD b % 8),
 // and each XCD has its own L2.  Neighbouring tiles share cache lines whenever
 // the innermost non-local mode has a small stride, so the tiles are cut into 8
@@ -404,8 +415,8 @@ struct TileRange {
   int first, end, step;
 };
 
-__device__ __forceinline__ TileRange xcd_tiles(int ntiles) {
-  const int G = gridDim.x, b = blockIdx.x;
+__device__ __forceinline__ TileRange xcd_tiles(int ntiles, int G) {
+  const int b = blockIdx.x;
   if (G % 8) return TileRange{b, ntiles, G};
   const int per = (ntiles + 7) / 8;
   const int x = b % 8;
@@ -418,11 +429,14 @@ template <typename T, int D, int K>
 __global__ void __launch_bounds__(sfamd::stage_threads(D, K)) k_stage_fwd(const C<T>* __restrict__ in, C<T>* __restrict__ out,
                                                              const C<T>* __restrict__ table, sfamd::StageArgs a) {
   using Gm = Geo<D, K>;
+  int64_t st[kNlPre];
+#pragma unroll
+  for (int u = 0; u < kNlPre; ++u) st[u] = a.nl_stride[u];
   extern __shared__ __align__(16) unsigned char smem[];
   C<T>* bufA = reinterpret_cast<C<T>*>(smem);
   C<T>* bufB = bufA + Gm::DK;
   C<T>* mats = bufB + Gm::DK;
-  const TileRange tr = xcd_tiles((int)a.ntiles);
+  const TileRange tr = xcd_tiles(a.ntiles, a.grid);
   const Mine<D, K> me(a);
   // the first tile's loads are in flight while the matrices are staged, and
   // each next tile's while the gates of the current one run
@@ -432,16 +446,16 @@ __global__ void __launch_bounds__(sfamd::stage_threads(D, K)) k_stage_fwd(const 
   const C<T>* img = table + a.img_fwd;
   C<T> iv[kStageBatch];
   image_load<T, Gm::NT>(img, a.mat_fwd, 0, iv);
-  fetch_tile<T, D, K>(in, tile_base<D>(tr.first < tr.end ? tr.first : 0, a), me, pre);
+  fetch_tile<T, D, K>(in, tile_base<D>(tr.first < tr.end ? tr.first : 0, a, st), me, pre);
   image_store<T, Gm::NT>(mats, a.mat_fwd, 0, iv);
   image_rest<T, Gm::NT>(img, a.mat_fwd, mats);
   for (int t = tr.first; t < tr.end; t += tr.step) {
-    const int64_t base = tile_base<D>(t, a);
+    const int64_t base = tile_base<D>(t, a, st);
 #pragma unroll
     for (int j = 0; j < Gm::PER; ++j)
       if (owns<D, K>(j)) bufA[threadIdx.x + j * Gm::NT] = pre[j];
     __syncthreads();
-    if (t + tr.step < tr.end) fetch_tile<T, D, K>(in, tile_base<D>(t + tr.step, a), me, pre);
+    if (t + tr.step < tr.end) fetch_tile<T, D, K>(in, tile_base<D>(t + tr.step, a, st), me, pre);
     C<T>* src = bufA;
     C<T>* dst = bufB;
     for (int gi = 0; gi < a.ngates - 1; ++gi) {
@@ -466,6 +480,9 @@ __global__ void __launch_bounds__(sfamd::stage_threads(D, K)) k_stage_adj(const 
                                                              const C<T>* __restrict__ table, sfamd::StageArgs a,
                                                              double* __restrict__ partial, int nwg) {
   using Gm = Geo<D, K>;
+  int64_t st[kNlPre];
+#pragma unroll
+  for (int u = 0; u < kNlPre; ++u) st[u] = a.nl_stride[u];
   extern __shared__ __align__(16) unsigned char smem[];
   // layout: [ngates + 1 psi tiles (every gate's input, then the last output)]
   //         [2 lam tiles][matrices + derivatives][grad slots [ngates][2][waves]]
@@ -476,7 +493,7 @@ __global__ void __launch_bounds__(sfamd::stage_threads(D, K)) k_stage_adj(const 
   double* gsl = reinterpret_cast<double*>(mats + a.mat_adj);
   constexpr int kWaves = Gm::NT / 64;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const TileRange tr = xcd_tiles((int)a.ntiles);
+  const TileRange tr = xcd_tiles(a.ntiles, a.grid);
   const Mine<D, K> me(a);
   // tile loads in flight while the matrices are staged (first tile) and while
   // the current tile's gates run (the next one)
@@ -487,7 +504,7 @@ __global__ void __launch_bounds__(sfamd::stage_threads(D, K)) k_stage_adj(const 
   C<T> iv[kStageBatch];
   image_load<T, Gm::NT>(img, a.mat_adj, 0, iv);
   {
-    const int64_t b = tile_base<D>(tr.first < tr.end ? tr.first : 0, a);
+    const int64_t b = tile_base<D>(tr.first < tr.end ? tr.first : 0, a, st);
     fetch_tile<T, D, K>(psi_in, b, me, pre_psi);
     fetch_tile<T, D, K>(lam_out, b, me, pre_lam);
   }
@@ -495,7 +512,7 @@ __global__ void __launch_bounds__(sfamd::stage_threads(D, K)) k_stage_adj(const 
   image_store<T, Gm::NT>(mats, a.mat_adj, 0, iv);
   image_rest<T, Gm::NT>(img, a.mat_adj, mats);
   for (int t = tr.first; t < tr.end; t += tr.step) {
-    const int64_t base = tile_base<D>(t, a);
+    const int64_t base = tile_base<D>(t, a, st);
 #pragma unroll
     for (int j = 0; j < Gm::PER; ++j) {
       if (owns<D, K>(j)) {
@@ -505,7 +522,7 @@ __global__ void __launch_bounds__(sfamd::stage_threads(D, K)) k_stage_adj(const 
     }
     __syncthreads();
     if (t + tr.step < tr.end) {
-      const int64_t b = tile_base<D>(t + tr.step, a);
+      const int64_t b = tile_base<D>(t + tr.step, a, st);
       fetch_tile<T, D, K>(psi_in, b, me, pre_psi);
       fetch_tile<T, D, K>(lam_out, b, me, pre_lam);
     }
@@ -520,7 +537,9 @@ __global__ void __launch_bounds__(sfamd::stage_threads(D, K)) k_stage_adj(const 
     C<T>* li = lamB;
     for (int gi = a.ngates - 1; gi >= 0; --gi) {
       const sfamd::StageGate& g = a.g[gi];
-      double g0 = 0.0, g1 = 0.0;
+      // per-thread parameter sums in the state's precision, reduced per wave
+      // and then added in double
+      T g0 = (T)0, g1 = (T)0;
       const C<T>* pin = tiles + gi * Gm::DK;
       const C<T>* pout = pin + Gm::DK;
       if (gi > 0)
@@ -533,8 +552,8 @@ __global__ void __launch_bounds__(sfamd::stage_threads(D, K)) k_stage_adj(const 
       g0 = wave_sum(g0);
       if (two) g1 = wave_sum(g1);
       if (lane == kWaveSumLane) {  // each wave owns its slot: deterministic, no atomics
-        gsl[(gi * 2 + 0) * kWaves + wave] += g0;
-        if (two) gsl[(gi * 2 + 1) * kWaves + wave] += g1;
+        gsl[(gi * 2 + 0) * kWaves + wave] += (double)g0;
+        if (two) gsl[(gi * 2 + 1) * kWaves + wave] += (double)g1;
       }
       __syncthreads();
       C<T>* tmp = lo_;
@@ -567,8 +586,10 @@ template <typename T, int D, int K>
 int launch_fwd(const void* in, void* out, const void* table, const sfamd::StageArgs& a, int grid, hipStream_t s) {
   const size_t lds = (size_t)sfamd::stage_lds_fwd(D, K, a.mat_fwd, sizeof(C<T>) == 8 ? SFAMD_C64 : SFAMD_C128);
   RET_IF(allow_lds(k_stage_fwd<T, D, K>, lds));
+  sfamd::StageArgs b = a;
+  b.grid = grid;
   hipLaunchKernelGGL((k_stage_fwd<T, D, K>), dim3(grid), dim3(Geo<D, K>::NT), lds, s, (const C<T>*)in, (C<T>*)out,
-                     (const C<T>*)table, a);
+                     (const C<T>*)table, b);
   return (int)hipGetLastError();
 }
 
@@ -578,8 +599,10 @@ int launch_adj(const void* psi, const void* lo, void* li, const void* table, con
   const size_t lds =
       (size_t)sfamd::stage_lds_adj(D, K, a.ngates, a.mat_adj, sizeof(C<T>) == 8 ? SFAMD_C64 : SFAMD_C128);
   RET_IF(allow_lds(k_stage_adj<T, D, K>, lds));
+  sfamd::StageArgs b = a;
+  b.grid = grid;
   hipLaunchKernelGGL((k_stage_adj<T, D, K>), dim3(grid), dim3(Geo<D, K>::NT), lds, s, (const C<T>*)psi,
-                     (const C<T>*)lo, (C<T>*)li, (const C<T>*)table, a, partial, nwg);
+                     (const C<T>*)lo, (C<T>*)li, (const C<T>*)table, b, partial, nwg);
   return (int)hipGetLastError();
 }
 

@@ -340,7 +340,7 @@ unsigned op_mask(const OpPlan& op) {
 // one of them (gates on disjoint modes commute) and the adjoint kernel's LDS
 // (one tile per gate plus two, and every gate's matrix + derivatives) fits
 void absorb(const std::vector<OpPlan>& ops, const std::vector<int>& rem, unsigned mask, int D, int Kc,
-            int64_t tile_bytes, int64_t esz, std::vector<int>* got) {
+            int64_t tile_bytes, int64_t esz, int64_t budget, std::vector<int>* got) {
   got->clear();
   unsigned blocked = 0;
   int64_t used = 3 * tile_bytes;  // the stage input and two cotangent tiles
@@ -349,7 +349,7 @@ void absorb(const std::vector<OpPlan>& ops, const std::vector<int>& rem, unsigne
     const unsigned om = op_mask(op);
     const int64_t cost = tile_bytes + (((stage_gate_elems(op.kind, D, true) + 1) & ~(int64_t)1) * esz) +
                          2 * (stage_threads(D, Kc) / 64) * 8;
-    if ((int)got->size() < kStageMaxGates && op_in(op, mask) && !(om & blocked) && used + cost <= kStageLdsBudget) {
+    if ((int)got->size() < kStageMaxGates && op_in(op, mask) && !(om & blocked) && used + cost <= budget) {
       got->push_back(i);
       used += cost;
     } else {
@@ -363,6 +363,10 @@ void plan_run(sfamd_prog* p, const std::vector<int>& run, int Kc) {
   const int n = p->n, D = p->D;
   const int64_t esz = elem_size(p->dtype);
   const int64_t tile_bytes = stage_tile(D, Kc) * esz;
+  // a launch has one workgroup per tile (up to the grid cap): when that is
+  // more than one per compute unit, two must fit in a CU's LDS at once
+  const int64_t ntiles = ipow(D, n - Kc);
+  const int64_t budget = ntiles > kComputeUnits ? kStageLdsPerCu / 2 - 512 : kStageLdsBudget;
   std::vector<int> rem = run;
   std::vector<unsigned> cands;
   // all Kc-subsets when few, else subsets grown from the next ops
@@ -396,16 +400,36 @@ void plan_run(sfamd_prog* p, const std::vector<int>& run, int Kc) {
       local.push_back(m);
     }
     unsigned bm = 0;
-    long bkey = -1;
-    for (unsigned m : local) {
-      absorb(p->ops, rem, m, D, Kc, tile_bytes, esz, &got);
-      // more gates first; then tiles that include the last modes (contiguous runs)
-      const long key = (long)got.size() * (1L << 26) + (long)m;
-      if (key > bkey) {
-        bkey = key;
-        bm = m;
-        best = got;
+    best.clear();
+    // the occupancy budget is a preference: a gate too large for it (compact
+    // beamsplitter tables at high cutoff) gets a stage of the full budget
+    for (int64_t bud : {budget, kStageLdsBudget}) {
+      long bkey = -1;
+      for (unsigned m : local) {
+        absorb(p->ops, rem, m, D, Kc, tile_bytes, esz, bud, &got);
+        // more gates first; then tiles that include the last modes (contiguous runs)
+        const long key = (long)got.size() * (1L << 26) + (long)m;
+        if (key > bkey) {
+          bkey = key;
+          bm = m;
+          best = got;
+        }
       }
+      if (!best.empty()) break;
+    }
+    if (best.empty()) {
+      // no stage holds the next operation: it runs on its own, unfused
+      Step one;
+      one.fused = 0;
+      one.op = rem.front();
+      one.K = 0;
+      one.diag = 0;
+      one.ops.push_back(rem.front());
+      memset(&one.args, 0, sizeof(one.args));
+      memset(&one.run, 0, sizeof(one.run));
+      p->steps.push_back(one);
+      rem.erase(rem.begin());
+      continue;
     }
     Step st;
     st.fused = 1;
@@ -425,7 +449,8 @@ void plan_run(sfamd_prog* p, const std::vector<int>& run, int Kc) {
         a.nl_stride[nlc++] = ipow(D, n - 1 - md);
     }
     a.nnl = nlc;
-    a.ntiles = ipow(D, nlc);
+    a.ntiles = (int32_t)ipow(D, nlc);
+    a.grid = 0;
     for (int t = 0; t < Kc; ++t) a.loc_stride[t] = ipow(D, n - 1 - lmodes[t]);
     int64_t mf = 0, ma = 0;
     for (int gi = 0; gi < a.ngates; ++gi) {

@@ -578,3 +578,32 @@ def test_batch_size_option_is_rejected_clearly():
     be = HIPFockBackend()
     with pytest.raises(NotImplementedError, match="batch_size"):
         be.begin_circuit(2, cutoff_dim=4, batch_size=3)
+
+
+def test_planner_handles_gates_too_large_for_the_occupancy_budget():
+    """At high cutoffs the compact beamsplitter tables of an adjoint stage
+    exceed the LDS a workgroup may take when a launch needs two per compute
+    unit; the planner then gives that stage the full budget (or runs the gate
+    unfused) instead of producing an empty stage.  Every operation appears in
+    exactly one step, in program order within each step."""
+    lib = _lib.load()
+    kinds = {"R": _lib.OP_ROTATION, "K": _lib.OP_KERR, "D": _lib.OP_DISPLACEMENT, "S": _lib.OP_SQUEEZE,
+             "BS": _lib.OP_BEAMSPLITTER}
+    rng = np.random.default_rng(5)
+    for n, D, fuse in [(4, 19, 2), (4, 20, 2), (4, 20, 3), (5, 16, 3), (6, 8, 3), (5, 8, 4)]:
+        ops = [(kinds[name], m[0], m[1] if len(m) > 1 else -1) for name, m, _ in circuits.layered(n, 1, rng)]
+        for dtype in (_lib.C64, _lib.C128):
+            st, h = _create(ops, n, D, dtype=dtype, fuse=fuse)
+            assert st == 0, (n, D, fuse)
+            info = _lib.ProgInfo()
+            assert lib.sfamd_prog_info(h, ctypes.byref(info)) == 0
+            seen = []
+            buf = (ctypes.c_int32 * len(ops))()
+            for s in range(info.nsteps):
+                k = lib.sfamd_prog_step_ops(h, s, buf, len(ops))
+                assert k >= 1
+                step = [buf[j] for j in range(k)]
+                assert step == sorted(step)
+                seen += step
+            assert sorted(seen) == list(range(len(ops))), (n, D, fuse)
+            assert lib.sfamd_prog_destroy(h) == 0
