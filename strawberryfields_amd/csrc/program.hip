@@ -386,7 +386,7 @@ void plan_run(sfamd_prog* p, const std::vector<int>& run, int Kc) {
       for (int u = t + 1; u < Kc; ++u) idx[u] = idx[u - 1] + 1;
     }
   }
-  std::vector<int> got, best;
+  std::vector<int> got, best, rem2, got2;
   while (!rem.empty()) {
     std::vector<unsigned> local = cands;
     if (local.empty()) {
@@ -407,8 +407,26 @@ void plan_run(sfamd_prog* p, const std::vector<int>& run, int Kc) {
       long bkey = -1;
       for (unsigned m : local) {
         absorb(p->ops, rem, m, D, Kc, tile_bytes, esz, bud, &got);
-        // more gates first; then tiles that include the last modes (contiguous runs)
-        const long key = (long)got.size() * (1L << 26) + (long)m;
+        // one step of lookahead: the gates this stage takes plus the most the
+        // next stage could take after it (on c3: 49 stages instead of 52);
+        // ties go to more gates now, then to tiles that include the last
+        // modes (contiguous runs)
+        long nxt = 0;
+        if (!got.empty() && local.size() > 1 && got.size() < rem.size()) {
+          rem2.clear();
+          size_t gi = 0;
+          for (int i : rem) {
+            if (gi < got.size() && got[gi] == i)
+              ++gi;
+            else
+              rem2.push_back(i);
+          }
+          for (unsigned m2 : local) {
+            absorb(p->ops, rem2, m2, D, Kc, tile_bytes, esz, bud, &got2);
+            if ((long)got2.size() > nxt) nxt = (long)got2.size();
+          }
+        }
+        const long key = (((long)got.size() + nxt) << 40) + ((long)got.size() << 26) + (long)m;
         if (key > bkey) {
           bkey = key;
           bm = m;
