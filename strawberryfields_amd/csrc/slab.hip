@@ -98,16 +98,21 @@ __device__ __forceinline__ void tile_load(C<T>* sh, const C<T>* __restrict__ g, 
     f4s* dst = (f4s*)sh;
     const int cpr = v.run * D == t.slab ? 0 : (int)(v.run * sizeof(C<T>) / 16);
     const int n = (int)(v.elems * sizeof(C<T>) / 16);
-    for (int c0 = threadIdx.x; c0 < n; c0 += kCopyBatch * blockDim.x) {
+    // uniform trip count; every load is issued (chunk index clamped into the
+    // tile) so that no branch separates them: conditional loads into the
+    // batch array made the compiler wait for each load before the next
+    for (int b0 = 0; b0 < n; b0 += kCopyBatch * (int)blockDim.x) {
       f4s r[kCopyBatch];
 #pragma unroll
       for (int k = 0; k < kCopyBatch; ++k) {
-        const int c = c0 + k * (int)blockDim.x;
-        if (c < n) r[k] = __builtin_nontemporal_load((const f4s*)(src + chunk_off<T>(t, c, cpr, D)));
+        const int c = b0 + (int)threadIdx.x + k * (int)blockDim.x;
+        r[k] = __builtin_nontemporal_load((const f4s*)(src + chunk_off<T>(t, c < n ? c : n - 1, cpr, D)));
       }
 #pragma unroll
-      for (int k = 0; k < kCopyBatch; ++k)
-        if (c0 + k * (int)blockDim.x < n) dst[c0 + k * blockDim.x] = r[k];
+      for (int k = 0; k < kCopyBatch; ++k) {
+        const int c = b0 + (int)threadIdx.x + k * (int)blockDim.x;
+        if (c < n) dst[c] = r[k];
+      }
     }
     return;
   }
