@@ -74,10 +74,14 @@ k_axis_mfma(const C* __restrict__ in, C* __restrict__ out, int64_t NF, int64_t R
   for (int64_t blk = (int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6); blk < nblk; blk += nwave) {
     const int64_t f = blk * 16 + c;
     const bool valid = f < NF;
-    const int64_t base = fiber_base(valid ? f : 0, R, D) + (int64_t)(g * Q) * R;
+    const int64_t fb = fiber_base(valid ? f : 0, R, D);
+    const int64_t base = fb + (int64_t)(g * Q) * R;
     C x[Q];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) x[q] = (valid && g * Q + q < D) ? in[base + q * R] : mk<float>(0.f, 0.f);
+    for (int q = 0; q < Q; ++q) {
+      const C v = in[fb + (int64_t)clamp_amp(g * Q + q, D) * R];
+      x[q] = (valid && g * Q + q < D) ? v : mk<float>(0.f, 0.f);
+    }
     f4 acc[Plan<D>::Mb];
     apply_operator<D>(A, x, acc);
     if (valid) {
@@ -111,13 +115,16 @@ k_axis_mfma_adj(const C* __restrict__ psi, const C* __restrict__ lo, C* __restri
     __asm__ volatile("" ::: "memory");
     const int64_t f = blk * 16 + c;
     const bool valid = f < NF;
-    const int64_t base = fiber_base(valid ? f : 0, R, D) + (int64_t)(g * Q) * R;
+    const int64_t fb = fiber_base(valid ? f : 0, R, D);
+    const int64_t base = fb + (int64_t)(g * Q) * R;
     C x[Q], l[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const bool in_range = valid && g * Q + q < D;
-      x[q] = in_range ? psi[base + q * R] : mk<float>(0.f, 0.f);
-      l[q] = in_range ? lo[base + q * R] : mk<float>(0.f, 0.f);
+      const int64_t o = fb + (int64_t)clamp_amp(g * Q + q, D) * R;
+      const C xv = psi[o], lv = lo[o];
+      x[q] = in_range ? xv : mk<float>(0.f, 0.f);
+      l[q] = in_range ? lv : mk<float>(0.f, 0.f);
     }
     f4 acc[MB];
     apply_op_lds<MB>(opl, l, acc);
@@ -213,10 +220,14 @@ __device__ __forceinline__ void bs_task(const C* __restrict__ in, C* __restrict_
   for (int b16 = threadIdx.x >> 6; b16 * 16 < nf; b16 += blockDim.x >> 6) {
     const bool valid = b16 * 16 + cl < nf;
     const int64_t f = f0 + (valid ? b16 * 16 + cl : 0);
-    const int64_t base = pair_base(f, geo, D) + first + (int64_t)(g * Q) * step;
+    const int64_t fb = pair_base(f, geo, D) + first;
+    const int64_t base = fb + (int64_t)(g * Q) * step;
     C x[Q];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) x[q] = (valid && g * Q + q < c) ? in[base + q * step] : mk<float>(0.f, 0.f);
+    for (int q = 0; q < Q; ++q) {
+      const C v = in[fb + (int64_t)clamp_amp(g * Q + q, c) * step];
+      x[q] = (valid && g * Q + q < c) ? v : mk<float>(0.f, 0.f);
+    }
     f4 acc[MB];
     apply_op<MB>(A, x, acc);
     if (valid) {
@@ -283,13 +294,16 @@ __device__ __forceinline__ void bs_adj_task(const C* __restrict__ psi, const C* 
     __asm__ volatile("" ::: "memory");  // operators stay in LDS (see k_axis_mfma_adj)
     const bool valid = b16 * 16 + cl < nf;
     const int64_t f = f0 + (valid ? b16 * 16 + cl : 0);
-    const int64_t base = pair_base(f, geo, D) + first + (int64_t)(g * Q) * step;
+    const int64_t fb = pair_base(f, geo, D) + first;
+    const int64_t base = fb + (int64_t)(g * Q) * step;
     C x[Q], l[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const bool in_range = valid && g * Q + q < c;
-      x[q] = in_range ? psi[base + q * step] : mk<float>(0.f, 0.f);
-      l[q] = in_range ? lo_[base + q * step] : mk<float>(0.f, 0.f);
+      const int64_t o = fb + (int64_t)clamp_amp(g * Q + q, c) * step;
+      const C xv = psi[o], lv = lo_[o];
+      x[q] = in_range ? xv : mk<float>(0.f, 0.f);
+      l[q] = in_range ? lv : mk<float>(0.f, 0.f);
     }
     f4 acc[MB];
     apply_op_lds<MB>(opl, l, acc);
@@ -352,6 +366,8 @@ k_bs_mfma_adj(const C* __restrict__ psi, const C* __restrict__ lo_, C* __restric
 //   sk == 1:  tau = 16 consecutive D x D slabs (contiguous); the slabs are
 //             staged in LDS ([tau][i][j], slab stride D*D + 1 against bank
 //             conflicts) and both passes run in place.
+constexpr int kStageCopy = 8;
+
 template <int D, bool STAGED>
 __global__ void __launch_bounds__(256)
 k_dm_dense_mfma(const C* __restrict__ in, C* __restrict__ out, int64_t N, int64_t sk, const C* __restrict__ M) {
@@ -373,7 +389,21 @@ k_dm_dense_mfma(const C* __restrict__ in, C* __restrict__ out, int64_t N, int64_
       const int64_t a0 = tile * 16;
       const int gs = (int)(nouter - a0 < 16 ? nouter - a0 : 16);
       const C* src = in + a0 * D * D;
-      for (int e = threadIdx.x; e < gs * D * D; e += blockDim.x) sh[(e / (D * D)) * PT + e % (D * D)] = src[e];
+      const int ne = gs * D * D;
+      // batches of kStageCopy loads in flight per thread (index clamped, store guarded)
+      for (int e0 = 0; e0 < ne; e0 += kStageCopy * (int)blockDim.x) {
+        C v[kStageCopy];
+#pragma unroll
+        for (int u = 0; u < kStageCopy; ++u) {
+          const int e = e0 + (int)threadIdx.x + u * (int)blockDim.x;
+          v[u] = src[e < ne ? e : ne - 1];
+        }
+#pragma unroll
+        for (int u = 0; u < kStageCopy; ++u) {
+          const int e = e0 + (int)threadIdx.x + u * (int)blockDim.x;
+          if (e < ne) sh[(e / (D * D)) * PT + e % (D * D)] = v[u];
+        }
+      }
       __syncthreads();
       // pass 1: fibers (tau, j), amplitudes i; pass 2: fibers (tau, i), amplitudes j
 #pragma unroll
@@ -387,8 +417,10 @@ k_dm_dense_mfma(const C* __restrict__ in, C* __restrict__ out, int64_t N, int64_
           const int st = pass == 0 ? D : 1;
           C x[Q];
 #pragma unroll
-          for (int q = 0; q < Q; ++q)
-            x[q] = (valid && g * Q + q < D) ? sh[base + (g * Q + q) * st] : mk<float>(0.f, 0.f);
+          for (int q = 0; q < Q; ++q) {
+            const C v = sh[base + clamp_amp(g * Q + q, D) * st];
+            x[q] = (valid && g * Q + q < D) ? v : mk<float>(0.f, 0.f);
+          }
           f4 acc[MB];
           if (pass == 0)
             apply_op<MB>(AU, x, acc);
@@ -414,8 +446,10 @@ k_dm_dense_mfma(const C* __restrict__ in, C* __restrict__ out, int64_t N, int64_
       for (int j = wave; j < D; j += nw) {
         C x[Q];
 #pragma unroll
-        for (int q = 0; q < Q; ++q)
-          x[q] = (valid && g * Q + q < D) ? in[col + (g * Q + q) * sb + j * sk] : mk<float>(0.f, 0.f);
+        for (int q = 0; q < Q; ++q) {
+          const C v = in[col + (int64_t)clamp_amp(g * Q + q, D) * sb + j * sk];
+          x[q] = (valid && g * Q + q < D) ? v : mk<float>(0.f, 0.f);
+        }
         f4 acc[MB];
         apply_op<MB>(AU, x, acc);
 #pragma unroll
@@ -427,7 +461,10 @@ k_dm_dense_mfma(const C* __restrict__ in, C* __restrict__ out, int64_t N, int64_
       for (int i = wave; i < D; i += nw) {
         C x[Q];
 #pragma unroll
-        for (int q = 0; q < Q; ++q) x[q] = g * Q + q < D ? sh[(i * D + g * Q + q) * 16 + c] : mk<float>(0.f, 0.f);
+        for (int q = 0; q < Q; ++q) {
+          const C v = sh[(i * D + clamp_amp(g * Q + q, D)) * 16 + c];
+          x[q] = g * Q + q < D ? v : mk<float>(0.f, 0.f);
+        }
         f4 acc[MB];
         apply_op<MB>(AC, x, acc);
         if (valid) {

@@ -47,13 +47,20 @@ __device__ __forceinline__ void load_op(F entry, float (&A)[MB][Blk<MB>::NS]) {
 }
 
 // entry of op(M) for an n x n row-major matrix; op: 0 -> M, 1 -> conj(M),
-// 2 -> M^H, 3 -> M^T
+// 2 -> M^H, 3 -> M^T; zero outside the matrix.  The load is unconditional
+// (indices clamped into the matrix) and the zero a select after it: with a
+// branch around each load, the compiler waits for every load of an unrolled
+// operator build before issuing the next.
 __device__ __forceinline__ C op_entry(const C* __restrict__ M, int n, int op, int i, int k) {
-  if (i >= n || k >= n) return mk<float>(0.f, 0.f);
-  C m = op >= 2 ? M[k * n + i] : M[i * n + k];
+  const int ic = i < n ? i : n - 1, kc = k < n ? k : n - 1;
+  C m = op >= 2 ? M[kc * n + ic] : M[ic * n + kc];
   if (op == 1 || op == 2) m.im = -m.im;
-  return m;
+  return (i < n && k < n) ? m : mk<float>(0.f, 0.f);
 }
+
+// index q of a group's amplitudes clamped to the last of n (for loads that
+// are always issued; the value is replaced by zero when q >= n)
+__device__ __forceinline__ int clamp_amp(int q, int n) { return q < n ? q : n - 1; }
 
 // acc = op x for this lane's 16-fiber block (x in group layout)
 template <int MB>
@@ -70,17 +77,32 @@ __device__ __forceinline__ void apply_op(const float (&A)[MB][Blk<MB>::NS], cons
 
 // Operators shared by every wave of a workgroup (adjoint kernels) live in LDS
 // in lane order, dst[(mb * NS + s) * 64 + lane]; the block's threads build them
-// together and the A operand of each MFMA is one conflict-free ds_read.
+// together and the A operand of each MFMA is one conflict-free ds_read.  A
+// thread computes its entries in batches of kOpBatch (every entry of a batch
+// is read, the element index clamped to the last) before storing them, so
+// that their loads are in flight together.
+constexpr int kOpBatch = 4;
 template <int MB, typename F>
 __device__ __forceinline__ void build_op_lds(F entry, float* dst) {
-  constexpr int Q = Blk<MB>::Q, NS = Blk<MB>::NS;
-  for (int e = threadIdx.x; e < MB * NS * 64; e += blockDim.x) {
-    const int lane = e & 63, slot = e >> 6;
-    const int mb = slot / NS, s = slot - mb * NS;
-    const int c = lane & 15, g = lane >> 4;
-    const int a = c & 1, b = s & 1;
-    const C m = entry((c >> 2) * Q + 2 * mb + ((c & 3) >> 1), g * Q + (s >> 1));
-    dst[e] = a == b ? m.re : (a ? m.im : -m.im);
+  constexpr int Q = Blk<MB>::Q, NS = Blk<MB>::NS, NE = MB * NS * 64;
+  for (int e0 = 0; e0 < NE; e0 += kOpBatch * (int)blockDim.x) {
+    float v[kOpBatch];
+#pragma unroll
+    for (int u = 0; u < kOpBatch; ++u) {
+      const int e = e0 + (int)threadIdx.x + u * (int)blockDim.x;
+      const int ec = e < NE ? e : NE - 1;
+      const int lane = ec & 63, slot = ec >> 6;
+      const int mb = slot / NS, s = slot - mb * NS;
+      const int c = lane & 15, g = lane >> 4;
+      const int a = c & 1, b = s & 1;
+      const C m = entry((c >> 2) * Q + 2 * mb + ((c & 3) >> 1), g * Q + (s >> 1));
+      v[u] = a == b ? m.re : (a ? m.im : -m.im);
+    }
+#pragma unroll
+    for (int u = 0; u < kOpBatch; ++u) {
+      const int e = e0 + (int)threadIdx.x + u * (int)blockDim.x;
+      if (e < NE) dst[e] = v[u];
+    }
   }
 }
 

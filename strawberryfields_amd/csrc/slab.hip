@@ -507,12 +507,17 @@ struct LossPol {
     pos0 = f.pos0;
     step = f.step;
   }
-  // out[i] = sum_l c[p0 + i][p0 + i + d][l] x[i + l]: entry (i, k = i + l)
+  // out[i] = sum_l c[p0 + i][p0 + i + d][l] x[i + l]: entry (i, k = i + l);
+  // zero outside 0 <= i <= k < n, the load always issued (indices clamped
+  // into that range, as in op_entry)
   __device__ Cf coef_entry(const float* cf, int di, int n, int i, int k) const {
-    if (i >= n || k >= n || k < i) return mk<float>(0.f, 0.f);
     const int d = di - (D - 1);
     const int p0 = d < 0 ? -d : 0;
-    return mk<float>(cf[((p0 * D) + p0 + d) * D + i * (D + 1) * D + (k - i)], 0.f);
+    const int ic = i < n ? i : n - 1;
+    const int l = k - i;
+    const int lc = l < 0 ? 0 : (l > n - 1 - ic ? n - 1 - ic : l);
+    const float c = cf[((p0 * D) + p0 + d) * D + ic * (D + 1) * D + lc];
+    return mk<float>((i < n && k < n && k >= i) ? c : 0.f, 0.f);
   }
   __device__ Cf entry(int di, int n, int i, int k) const { return coef_entry(coef, di, n, i, k); }
   __device__ Cf entry_h(int di, int n, int i, int k) const { return coef_entry(coef, di, n, k, i); }
@@ -533,10 +538,13 @@ __device__ __forceinline__ void slab_block(Cf* sh, const P& pol, int b, int n, c
     const bool valid = phi < per;
     int pos0, step;
     pol.fiber(b, valid ? phi : 0, t, v, pos0, step);
-    pos0 += g * Q * step;
     Cf x[Q];
 #pragma unroll
-    for (int q = 0; q < Q; ++q) x[q] = (valid && g * Q + q < n) ? sh[pos0 + q * step] : mk<float>(0.f, 0.f);
+    for (int q = 0; q < Q; ++q) {
+      const Cf xv = sh[pos0 + sfamd::mf::clamp_amp(g * Q + q, n) * step];
+      x[q] = (valid && g * Q + q < n) ? xv : mk<float>(0.f, 0.f);
+    }
+    pos0 += g * Q * step;
     f4 acc[MB];
     sfamd::mf::apply_op<MB>(A, x, acc);
     if (valid) {
@@ -597,14 +605,16 @@ __device__ __forceinline__ void slab_block_adj(const Cf* shx, Cf* shl, const P& 
     const bool valid = phi < per;
     int pos0, step;
     pol.fiber(b, valid ? phi : 0, t, v, pos0, step);
-    pos0 += gq * step;
     Cf x[Q], l[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       const bool in_range = valid && gq + q < n;
-      l[q] = in_range ? shl[pos0 + q * step] : mk<float>(0.f, 0.f);
-      x[q] = in_range ? shx[pos0 + q * step] : mk<float>(0.f, 0.f);
+      const int o = pos0 + sfamd::mf::clamp_amp(gq + q, n) * step;
+      const Cf lv = shl[o], xv = shx[o];
+      l[q] = in_range ? lv : mk<float>(0.f, 0.f);
+      x[q] = in_range ? xv : mk<float>(0.f, 0.f);
     }
+    pos0 += gq * step;
     f4 acc[MB];
     sfamd::mf::apply_op<MB>(A0, x, acc);
 #pragma unroll
