@@ -14,11 +14,11 @@
 // the register plan: lane l = 16 g + c (group g = 0..3, column c = 0..15)
 // supplies A[c][g] and B[g][c] and receives C[4 g + j][c], j = 0..3.  Columns
 // c are fibers, so each lane loads and stores amplitudes of one fiber only:
-// group g owns amplitudes k = g Q .. g Q + Q - 1 (Q = 2 Mb) for input and
-// output alike.  The real rows and columns are ordered so that this holds
-// without any data exchange:
-//   row   16 mb + c  <->  amplitude (c / 4) Q + 2 mb + (c % 4) / 2, part c % 2
-//   k-step s, group g  <->  amplitude g Q + s / 2, part s % 2.
+// group g owns amplitudes k = g P .. g P + P - 1 (P = ceil(D / 4)) in its
+// Q = 2 Mb slots for input and output alike.  The real rows and columns are
+// ordered so that this holds without any data exchange (mfma_ops.h):
+//   row   16 mb + c  <->  slot 2 mb + (c % 4) / 2 of group c / 4, part c % 2
+//   k-step s, group g  <->  slot s / 2 of group g, part s % 2.
 // The gate's real matrix lives in Mb * 4 Mb registers per lane (36 at D = 20)
 // for the whole launch.  Any axis stride works: 16 consecutive fibers are 16
 // consecutive inner positions (coalesced runs of 16 amplitudes) when the
@@ -36,11 +36,12 @@ struct Plan {
   static constexpr int Mb = (2 * D + 15) / 16;  // 16-row blocks
   static constexpr int Q = Blk<Mb>::Q;
   static constexpr int NS = Blk<Mb>::NS;
+  static constexpr int P = group_amps(D);  // amplitudes per lane group
 };
 
 template <int D>
 __device__ __forceinline__ void load_operator(const C* __restrict__ M, int op, float (&A)[Plan<D>::Mb][Plan<D>::NS]) {
-  load_op<Plan<D>::Mb>([&](int i, int k) { return op_entry(M, D, op, i, k); }, A);
+  load_op<Plan<D>::Mb>([&](int i, int k) { return op_entry(M, D, op, i, k); }, A, Plan<D>::P);
 }
 
 template <int D>
@@ -64,7 +65,7 @@ __device__ __forceinline__ int64_t fiber_base(int64_t f, int64_t R, int D) {
 template <int D>
 __global__ void __launch_bounds__(256)
 k_axis_mfma(const C* __restrict__ in, C* __restrict__ out, int64_t NF, int64_t R, const C* __restrict__ M, int op) {
-  constexpr int Q = Plan<D>::Q;
+  constexpr int Q = Plan<D>::Q, P = Plan<D>::P;
   float A[Plan<D>::Mb][Plan<D>::NS];
   load_operator<D>(M, op, A);
   const int lane = threadIdx.x & 63;
@@ -75,19 +76,21 @@ k_axis_mfma(const C* __restrict__ in, C* __restrict__ out, int64_t NF, int64_t R
     const int64_t f = blk * 16 + c;
     const bool valid = f < NF;
     const int64_t fb = fiber_base(valid ? f : 0, R, D);
-    const int64_t base = fb + (int64_t)(g * Q) * R;
     C x[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      const C v = in[fb + (int64_t)clamp_amp(g * Q + q, D) * R];
-      x[q] = (valid && g * Q + q < D) ? v : mk<float>(0.f, 0.f);
+      x[q] = mk<float>(0.f, 0.f);
+      if (q >= P) continue;
+      const int a = g * P + q;
+      const C v = in[fb + (int64_t)clamp_amp(a, D) * R];
+      if (valid && a < D) x[q] = v;
     }
     f4 acc[Plan<D>::Mb];
     apply_operator<D>(A, x, acc);
     if (valid) {
 #pragma unroll
-      for (int q = 0; q < Q; ++q)
-        if (g * Q + q < D) out[base + q * R] = acc_amp<D>(acc, q);
+      for (int q = 0; q < P; ++q)
+        if (g * P + q < D) out[fb + (int64_t)(g * P + q) * R] = acc_amp<D>(acc, q);
     }
   }
 }
@@ -98,11 +101,11 @@ __global__ void __launch_bounds__(256)
 k_axis_mfma_adj(const C* __restrict__ psi, const C* __restrict__ lo, C* __restrict__ li, int64_t NF, int64_t R,
                 const C* __restrict__ M, const C* __restrict__ dM0, const C* __restrict__ dM1, int conj,
                 double* __restrict__ partial, int nwg, int zp1) {
-  constexpr int MB = Plan<D>::Mb, Q = Plan<D>::Q, OPN = MB * Plan<D>::NS * 64;
+  constexpr int MB = Plan<D>::Mb, Q = Plan<D>::Q, P = Plan<D>::P, OPN = MB * Plan<D>::NS * 64;
   __shared__ float opl[3 * OPN];
-  build_op_lds<MB>([&](int i, int k) { return op_entry(M, D, conj ? 3 : 2, i, k); }, opl);
-  build_op_lds<MB>([&](int i, int k) { return op_entry(dM0, D, conj ? 1 : 0, i, k); }, opl + OPN);
-  build_op_lds<MB>([&](int i, int k) { return op_entry(dM1, D, conj ? 1 : 0, i, k); }, opl + 2 * OPN);
+  build_op_lds<MB>([&](int i, int k) { return op_entry(M, D, conj ? 3 : 2, i, k); }, opl, P);
+  build_op_lds<MB>([&](int i, int k) { return op_entry(dM0, D, conj ? 1 : 0, i, k); }, opl + OPN, P);
+  build_op_lds<MB>([&](int i, int k) { return op_entry(dM1, D, conj ? 1 : 0, i, k); }, opl + 2 * OPN, P);
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int c = lane & 15, g = lane >> 4;
@@ -116,22 +119,25 @@ k_axis_mfma_adj(const C* __restrict__ psi, const C* __restrict__ lo, C* __restri
     const int64_t f = blk * 16 + c;
     const bool valid = f < NF;
     const int64_t fb = fiber_base(valid ? f : 0, R, D);
-    const int64_t base = fb + (int64_t)(g * Q) * R;
     C x[Q], l[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      const bool in_range = valid && g * Q + q < D;
-      const int64_t o = fb + (int64_t)clamp_amp(g * Q + q, D) * R;
+      x[q] = l[q] = mk<float>(0.f, 0.f);
+      if (q >= P) continue;
+      const int a = g * P + q;
+      const int64_t o = fb + (int64_t)clamp_amp(a, D) * R;
       const C xv = psi[o], lv = lo[o];
-      x[q] = in_range ? xv : mk<float>(0.f, 0.f);
-      l[q] = in_range ? lv : mk<float>(0.f, 0.f);
+      if (valid && a < D) {
+        x[q] = xv;
+        l[q] = lv;
+      }
     }
     f4 acc[MB];
     apply_op_lds<MB>(opl, l, acc);
     if (valid) {
 #pragma unroll
-      for (int q = 0; q < Q; ++q)
-        if (g * Q + q < D) li[base + q * R] = acc_at<MB>(acc, q);
+      for (int q = 0; q < P; ++q)
+        if (g * P + q < D) li[fb + (int64_t)(g * P + q) * R] = acc_at<MB>(acc, q);
     }
     float s0 = 0.f, s1 = 0.f;
     apply_op_lds<MB>(opl + OPN, x, acc);
@@ -209,8 +215,9 @@ __device__ __forceinline__ void bs_task(const C* __restrict__ in, C* __restrict_
                                         int64_t sp, int64_t sq, const C* __restrict__ bb, int c, int lo, int s,
                                         int conj, int diff, int64_t f0) {
   constexpr int Q = Blk<MB>::Q;
+  const int P = group_amps(c);
   float A[MB][Blk<MB>::NS];
-  load_op<MB>([&](int i, int k) { return op_entry(bb, c, conj ? 1 : 0, i, k); }, A);
+  load_op<MB>([&](int i, int k) { return op_entry(bb, c, conj ? 1 : 0, i, k); }, A, P);
   const int lane = threadIdx.x & 63;
   const int cl = lane & 15, g = lane >> 4;
   const int64_t step = block_step(sp, sq, diff);
@@ -221,19 +228,19 @@ __device__ __forceinline__ void bs_task(const C* __restrict__ in, C* __restrict_
     const bool valid = b16 * 16 + cl < nf;
     const int64_t f = f0 + (valid ? b16 * 16 + cl : 0);
     const int64_t fb = pair_base(f, geo, D) + first;
-    const int64_t base = fb + (int64_t)(g * Q) * step;
     C x[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      const C v = in[fb + (int64_t)clamp_amp(g * Q + q, c) * step];
-      x[q] = (valid && g * Q + q < c) ? v : mk<float>(0.f, 0.f);
+      const int a = g * P + q;
+      const C v = in[fb + (int64_t)clamp_amp(a, c) * step];
+      x[q] = (valid && q < P && a < c) ? v : mk<float>(0.f, 0.f);
     }
     f4 acc[MB];
     apply_op<MB>(A, x, acc);
     if (valid) {
 #pragma unroll
       for (int q = 0; q < Q; ++q)
-        if (g * Q + q < c) out[base + q * step] = acc_at<MB>(acc, q);
+        if (q < P && g * P + q < c) out[fb + (int64_t)(g * P + q) * step] = acc_at<MB>(acc, q);
     }
   }
 }
@@ -272,9 +279,10 @@ __device__ __forceinline__ void bs_adj_task(const C* __restrict__ psi, const C* 
                                             int conj, int wa, int wb, int diff, int64_t f0, float* opl, double& g0,
                                             double& g1) {
   constexpr int Q = Blk<MB>::Q, OPN = MB * Blk<MB>::NS * 64;
+  const int P = group_amps(c);
   __syncthreads();  // the previous task's operators are no longer read
-  build_op_lds<MB>([&](int i, int k) { return op_entry(bb, c, conj ? 3 : 2, i, k); }, opl);
-  build_op_lds<MB>([&](int i, int k) { return op_entry(db, c, conj ? 1 : 0, i, k); }, opl + OPN);
+  build_op_lds<MB>([&](int i, int k) { return op_entry(bb, c, conj ? 3 : 2, i, k); }, opl, P);
+  build_op_lds<MB>([&](int i, int k) { return op_entry(db, c, conj ? 1 : 0, i, k); }, opl + OPN, P);
   const float sg = conj ? -1.f : 1.f;
   build_op_lds<MB>(
       [&](int i, int k) {
@@ -282,7 +290,7 @@ __device__ __forceinline__ void bs_adj_task(const C* __restrict__ psi, const C* 
         const float w = sg * (float)(wa * (lo + k) + wb * (lo + i));
         return mk<float>(-w * m.im, w * m.re);
       },
-      opl + 2 * OPN);
+      opl + 2 * OPN, P);
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int cl = lane & 15, g = lane >> 4;
@@ -295,12 +303,12 @@ __device__ __forceinline__ void bs_adj_task(const C* __restrict__ psi, const C* 
     const bool valid = b16 * 16 + cl < nf;
     const int64_t f = f0 + (valid ? b16 * 16 + cl : 0);
     const int64_t fb = pair_base(f, geo, D) + first;
-    const int64_t base = fb + (int64_t)(g * Q) * step;
     C x[Q], l[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      const bool in_range = valid && g * Q + q < c;
-      const int64_t o = fb + (int64_t)clamp_amp(g * Q + q, c) * step;
+      const int a = g * P + q;
+      const bool in_range = valid && q < P && a < c;
+      const int64_t o = fb + (int64_t)clamp_amp(a, c) * step;
       const C xv = psi[o], lv = lo_[o];
       x[q] = in_range ? xv : mk<float>(0.f, 0.f);
       l[q] = in_range ? lv : mk<float>(0.f, 0.f);
@@ -310,7 +318,7 @@ __device__ __forceinline__ void bs_adj_task(const C* __restrict__ psi, const C* 
     if (valid) {
 #pragma unroll
       for (int q = 0; q < Q; ++q)
-        if (g * Q + q < c) li[base + q * step] = acc_at<MB>(acc, q);
+        if (q < P && g * P + q < c) li[fb + (int64_t)(g * P + q) * step] = acc_at<MB>(acc, q);
     }
     float s0 = 0.f, s1 = 0.f;
     apply_op_lds<MB>(opl + OPN, x, acc);
@@ -371,7 +379,7 @@ constexpr int kStageCopy = 8;
 template <int D, bool STAGED>
 __global__ void __launch_bounds__(256)
 k_dm_dense_mfma(const C* __restrict__ in, C* __restrict__ out, int64_t N, int64_t sk, const C* __restrict__ M) {
-  constexpr int MB = Plan<D>::Mb, Q = Plan<D>::Q;
+  constexpr int MB = Plan<D>::Mb, Q = Plan<D>::Q, P = Plan<D>::P;
   constexpr int PT = D * D + 1;
   __shared__ C sh[STAGED ? 16 * PT : D * D * 16];
   float AU[MB][Plan<D>::NS], AC[MB][Plan<D>::NS];
@@ -418,8 +426,10 @@ k_dm_dense_mfma(const C* __restrict__ in, C* __restrict__ out, int64_t N, int64_
           C x[Q];
 #pragma unroll
           for (int q = 0; q < Q; ++q) {
-            const C v = sh[base + clamp_amp(g * Q + q, D) * st];
-            x[q] = (valid && g * Q + q < D) ? v : mk<float>(0.f, 0.f);
+            x[q] = mk<float>(0.f, 0.f);
+            if (q >= P) continue;
+            const C v = sh[base + clamp_amp(g * P + q, D) * st];
+            if (valid && g * P + q < D) x[q] = v;
           }
           f4 acc[MB];
           if (pass == 0)
@@ -428,8 +438,8 @@ k_dm_dense_mfma(const C* __restrict__ in, C* __restrict__ out, int64_t N, int64_
             apply_op<MB>(AC, x, acc);
           if (valid) {
 #pragma unroll
-            for (int q = 0; q < Q; ++q)
-              if (g * Q + q < D) sh[base + (g * Q + q) * st] = acc_at<MB>(acc, q);
+            for (int q = 0; q < P; ++q)
+              if (g * P + q < D) sh[base + (g * P + q) * st] = acc_at<MB>(acc, q);
           }
         }
         __syncthreads();
@@ -447,14 +457,16 @@ k_dm_dense_mfma(const C* __restrict__ in, C* __restrict__ out, int64_t N, int64_
         C x[Q];
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-          const C v = in[col + (int64_t)clamp_amp(g * Q + q, D) * sb + j * sk];
-          x[q] = (valid && g * Q + q < D) ? v : mk<float>(0.f, 0.f);
+          x[q] = mk<float>(0.f, 0.f);
+          if (q >= P) continue;
+          const C v = in[col + (int64_t)clamp_amp(g * P + q, D) * sb + j * sk];
+          if (valid && g * P + q < D) x[q] = v;
         }
         f4 acc[MB];
         apply_op<MB>(AU, x, acc);
 #pragma unroll
-        for (int q = 0; q < Q; ++q)
-          if (g * Q + q < D) sh[((g * Q + q) * D + j) * 16 + c] = acc_at<MB>(acc, q);
+        for (int q = 0; q < P; ++q)
+          if (g * P + q < D) sh[((g * P + q) * D + j) * 16 + c] = acc_at<MB>(acc, q);
       }
       __syncthreads();
       // pass 2: block i, amplitudes j from LDS -> HBM
@@ -462,15 +474,17 @@ k_dm_dense_mfma(const C* __restrict__ in, C* __restrict__ out, int64_t N, int64_
         C x[Q];
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
-          const C v = sh[(i * D + clamp_amp(g * Q + q, D)) * 16 + c];
-          x[q] = g * Q + q < D ? v : mk<float>(0.f, 0.f);
+          x[q] = mk<float>(0.f, 0.f);
+          if (q >= P) continue;
+          const C v = sh[(i * D + clamp_amp(g * P + q, D)) * 16 + c];
+          if (g * P + q < D) x[q] = v;
         }
         f4 acc[MB];
         apply_op<MB>(AC, x, acc);
         if (valid) {
 #pragma unroll
-          for (int q = 0; q < Q; ++q)
-            if (g * Q + q < D) out[col + i * sb + (g * Q + q) * sk] = acc_at<MB>(acc, q);
+          for (int q = 0; q < P; ++q)
+            if (g * P + q < D) out[col + i * sb + (int64_t)(g * P + q) * sk] = acc_at<MB>(acc, q);
         }
       }
       __syncthreads();

@@ -6,10 +6,15 @@
 // padded to 16 MB rows and columns.  The instruction's operand layouts fix the
 // register plan: lane l = 16 g + c supplies A[c][g] and B[g][c] and receives
 // C[4 g + j][c].  Columns c are fibers, so a lane touches amplitudes of one
-// fiber only, and group g owns amplitudes g Q .. g Q + Q - 1 (Q = 2 MB) of
-// both input and output:
-//   row 16 mb + c      <->  amplitude (c / 4) Q + 2 mb + (c % 4) / 2, part c % 2
-//   k-step s, group g  <->  amplitude g Q + s / 2, part s % 2.
+// fiber only.  Each lane group g holds Q = 2 MB slots of input and output;
+// for an operator of size n, group g owns amplitudes g P .. g P + P - 1 in
+// slots 0 .. P - 1 (P = ceil(n / 4) <= Q), and slots P .. Q - 1 are padding
+// (zero rows and columns):
+//   row 16 mb + c      <->  slot 2 mb + (c % 4) / 2 of group c / 4, part c % 2
+//   k-step s, group g  <->  slot s / 2 of group g, part s % 2,
+// slot t of group g  <->  amplitude g P + t (t < P).  Spreading the amplitudes
+// evenly over the four groups leaves at most three lanes' slots without an
+// amplitude (instead of 4 Q - n), so almost every load and store carries one.
 #pragma once
 
 #include "common.h"
@@ -24,16 +29,18 @@ using C = cpx<float>;
 // Register plan for an operator of up to 8 MB complex rows and columns.
 template <int MB>
 struct Blk {
-  static constexpr int Q = 2 * MB;   // amplitudes per lane group
+  static constexpr int Q = 2 * MB;   // slots per lane group
   static constexpr int NS = 4 * MB;  // k-steps (4 real columns each)
 };
 
-// this lane's A operands of the real form of a complex operator given by
-// entry(i, k) (row i = output amplitude, column k = input amplitude; zero
-// outside the operator): A[mb][s]
+// amplitudes per lane group for an operator of size n
+__host__ __device__ constexpr int group_amps(int n) { return (n + 3) / 4; }
+
+// this lane's A operands of the real form of a complex operator of size n
+// given by entry(i, k) (row i = output amplitude, column k = input amplitude;
+// zero outside the operator), P = group_amps(n): A[mb][s]
 template <int MB, typename F>
-__device__ __forceinline__ void load_op(F entry, float (&A)[MB][Blk<MB>::NS]) {
-  constexpr int Q = Blk<MB>::Q;
+__device__ __forceinline__ void load_op(F entry, float (&A)[MB][Blk<MB>::NS], int P) {
   const int lane = threadIdx.x & 63;
   const int c = lane & 15, g = lane >> 4;
 #pragma unroll
@@ -41,8 +48,10 @@ __device__ __forceinline__ void load_op(F entry, float (&A)[MB][Blk<MB>::NS]) {
 #pragma unroll
     for (int s = 0; s < Blk<MB>::NS; ++s) {
       const int a = c & 1, b = s & 1;
-      const C m = entry((c >> 2) * Q + 2 * mb + ((c & 3) >> 1), g * Q + (s >> 1));
-      A[mb][s] = a == b ? m.re : (a ? m.im : -m.im);
+      const int tr = 2 * mb + ((c & 3) >> 1), tc = s >> 1;
+      const C m = entry((c >> 2) * P + tr, g * P + tc);
+      const float v = a == b ? m.re : (a ? m.im : -m.im);
+      A[mb][s] = (tr < P && tc < P) ? v : 0.f;
     }
 }
 
@@ -58,9 +67,9 @@ __device__ __forceinline__ C op_entry(const C* __restrict__ M, int n, int op, in
   return (i < n && k < n) ? m : mk<float>(0.f, 0.f);
 }
 
-// index q of a group's amplitudes clamped to the last of n (for loads that
-// are always issued; the value is replaced by zero when q >= n)
-__device__ __forceinline__ int clamp_amp(int q, int n) { return q < n ? q : n - 1; }
+// amplitude index a clamped to the last of n (for loads that are always
+// issued; the value is replaced by zero when the slot holds no amplitude)
+__device__ __forceinline__ int clamp_amp(int a, int n) { return a < n ? a : n - 1; }
 
 // acc = op x for this lane's 16-fiber block (x in group layout)
 template <int MB>
@@ -83,8 +92,8 @@ __device__ __forceinline__ void apply_op(const float (&A)[MB][Blk<MB>::NS], cons
 // that their loads are in flight together.
 constexpr int kOpBatch = 4;
 template <int MB, typename F>
-__device__ __forceinline__ void build_op_lds(F entry, float* dst) {
-  constexpr int Q = Blk<MB>::Q, NS = Blk<MB>::NS, NE = MB * NS * 64;
+__device__ __forceinline__ void build_op_lds(F entry, float* dst, int P) {
+  constexpr int NS = Blk<MB>::NS, NE = MB * NS * 64;
   for (int e0 = 0; e0 < NE; e0 += kOpBatch * (int)blockDim.x) {
     float v[kOpBatch];
 #pragma unroll
@@ -95,8 +104,10 @@ __device__ __forceinline__ void build_op_lds(F entry, float* dst) {
       const int mb = slot / NS, s = slot - mb * NS;
       const int c = lane & 15, g = lane >> 4;
       const int a = c & 1, b = s & 1;
-      const C m = entry((c >> 2) * Q + 2 * mb + ((c & 3) >> 1), g * Q + (s >> 1));
-      v[u] = a == b ? m.re : (a ? m.im : -m.im);
+      const int tr = 2 * mb + ((c & 3) >> 1), tc = s >> 1;
+      const C m = entry((c >> 2) * P + tr, g * P + tc);
+      const float x = a == b ? m.re : (a ? m.im : -m.im);
+      v[u] = (tr < P && tc < P) ? x : 0.f;
     }
 #pragma unroll
     for (int u = 0; u < kOpBatch; ++u) {
@@ -120,7 +131,7 @@ __device__ __forceinline__ void apply_op_lds(const float* opl, const C (&x)[Blk<
   }
 }
 
-// amplitude q of this lane's group from the accumulators
+// slot q of this lane's group from the accumulators
 template <int MB>
 __device__ __forceinline__ C acc_at(const f4 (&acc)[MB], int q) {
   const f4 v = acc[q >> 1];

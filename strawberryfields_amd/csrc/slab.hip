@@ -529,7 +529,8 @@ template <int MB, int D, class P>
 __device__ __forceinline__ void slab_block(Cf* sh, const P& pol, int b, int n, const Tile& t, const View& v) {
   constexpr int Q = Blk<MB>::Q;
   float A[MB][Blk<MB>::NS];
-  sfamd::mf::load_op<MB>([&](int i, int k) { return pol.entry(b, n, i, k); }, A);
+  const int pa = sfamd::mf::group_amps(n);  // amplitudes per lane group
+  sfamd::mf::load_op<MB>([&](int i, int k) { return pol.entry(b, n, i, k); }, A, pa);
   const int per = pol.per_block(t, v);
   const int lane = threadIdx.x & 63;
   const int cl = lane & 15, g = lane >> 4;
@@ -541,16 +542,17 @@ __device__ __forceinline__ void slab_block(Cf* sh, const P& pol, int b, int n, c
     Cf x[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      const Cf xv = sh[pos0 + sfamd::mf::clamp_amp(g * Q + q, n) * step];
-      x[q] = (valid && g * Q + q < n) ? xv : mk<float>(0.f, 0.f);
+      const int a = g * pa + q;
+      const Cf xv = sh[pos0 + sfamd::mf::clamp_amp(a, n) * step];
+      x[q] = (valid && q < pa && a < n) ? xv : mk<float>(0.f, 0.f);
     }
-    pos0 += g * Q * step;
+    pos0 += g * pa * step;
     f4 acc[MB];
     sfamd::mf::apply_op<MB>(A, x, acc);
     if (valid) {
 #pragma unroll
       for (int q = 0; q < Q; ++q)
-        if (g * Q + q < n) sh[pos0 + q * step] = sfamd::mf::acc_at<MB>(acc, q);
+        if (q < pa && g * pa + q < n) sh[pos0 + q * step] = sfamd::mf::acc_at<MB>(acc, q);
     }
   }
 }
@@ -593,12 +595,13 @@ __device__ __forceinline__ void slab_block_adj(const Cf* shx, Cf* shl, const P& 
                                                const View& v, double& g0, double& g1) {
   constexpr int Q = Blk<MB>::Q;
   float A0[MB][Blk<MB>::NS], A1[MB][Blk<MB>::NS], AH[MB][Blk<MB>::NS];
-  sfamd::mf::load_op<MB>([&](int i, int k) { return pol.dentry0(b, n, i, k); }, A0);
-  if (P::kTwoParams) sfamd::mf::load_op<MB>([&](int i, int k) { return pol.dentry1(b, n, i, k); }, A1);
-  sfamd::mf::load_op<MB>([&](int i, int k) { return pol.entry_h(b, n, i, k); }, AH);
+  const int pa = sfamd::mf::group_amps(n);  // amplitudes per lane group
+  sfamd::mf::load_op<MB>([&](int i, int k) { return pol.dentry0(b, n, i, k); }, A0, pa);
+  if (P::kTwoParams) sfamd::mf::load_op<MB>([&](int i, int k) { return pol.dentry1(b, n, i, k); }, A1, pa);
+  sfamd::mf::load_op<MB>([&](int i, int k) { return pol.entry_h(b, n, i, k); }, AH, pa);
   const int per = pol.per_block(t, v);
   const int lane = threadIdx.x & 63;
-  const int cl = lane & 15, gq = (lane >> 4) * Q;
+  const int cl = lane & 15, gq = (lane >> 4) * pa;
   float s0 = 0.f, s1 = 0.f;
   for (int fb = 0; fb * 16 < per; ++fb) {
     const int phi = fb * 16 + cl;
@@ -608,7 +611,7 @@ __device__ __forceinline__ void slab_block_adj(const Cf* shx, Cf* shl, const P& 
     Cf x[Q], l[Q];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-      const bool in_range = valid && gq + q < n;
+      const bool in_range = valid && q < pa && gq + q < n;
       const int o = pos0 + sfamd::mf::clamp_amp(gq + q, n) * step;
       const Cf lv = shl[o], xv = shx[o];
       l[q] = in_range ? lv : mk<float>(0.f, 0.f);
@@ -628,7 +631,7 @@ __device__ __forceinline__ void slab_block_adj(const Cf* shx, Cf* shl, const P& 
     if (valid) {
 #pragma unroll
       for (int q = 0; q < Q; ++q)
-        if (gq + q < n) shl[pos0 + q * step] = sfamd::mf::acc_at<MB>(acc, q);
+        if (q < pa && gq + q < n) shl[pos0 + q * step] = sfamd::mf::acc_at<MB>(acc, q);
     }
   }
   g0 += (double)s0;
