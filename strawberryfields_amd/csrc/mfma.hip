@@ -47,7 +47,7 @@ __device__ __forceinline__ void load_operator(const C* __restrict__ M, int op, f
 template <int D>
 __device__ __forceinline__ void apply_operator(const float (&A)[Plan<D>::Mb][Plan<D>::NS], const C (&x)[Plan<D>::Q],
                                                f4 (&acc)[Plan<D>::Mb]) {
-  apply_op<Plan<D>::Mb>(A, x, acc);
+  apply_op<Plan<D>::Mb>(A, x, acc, Plan<D>::P);
 }
 
 template <int D>
@@ -133,17 +133,17 @@ k_axis_mfma_adj(const C* __restrict__ psi, const C* __restrict__ lo, C* __restri
       }
     }
     f4 acc[MB];
-    apply_op_lds<MB>(opl, l, acc);
+    apply_op_lds<MB>(opl, l, acc, P);
     if (valid) {
 #pragma unroll
       for (int q = 0; q < P; ++q)
         if (g * P + q < D) li[fb + (int64_t)(g * P + q) * R] = acc_at<MB>(acc, q);
     }
     float s0 = 0.f, s1 = 0.f;
-    apply_op_lds<MB>(opl + OPN, x, acc);
+    apply_op_lds<MB>(opl + OPN, x, acc, P);
 #pragma unroll
     for (int q = 0; q < Q; ++q) s0 += re_conj_mul(l[q], acc_at<MB>(acc, q));
-    apply_op_lds<MB>(opl + 2 * OPN, x, acc);
+    apply_op_lds<MB>(opl + 2 * OPN, x, acc, P);
 #pragma unroll
     for (int q = 0; q < Q; ++q) s1 += re_conj_mul(l[q], acc_at<MB>(acc, q));
     g0 += (double)s0;
@@ -208,14 +208,22 @@ __device__ __forceinline__ int64_t block_first(int64_t sp, int64_t sq, int lo, i
 }
 __device__ __forceinline__ int64_t block_step(int64_t sp, int64_t sq, int diff) { return diff ? sp + sq : sp - sq; }
 
+// A block of c amplitudes has P = group_amps(c) = Q - 1 or Q amplitudes per
+// lane group (Q = 2 MB, MB = ceil(2c / 16)); the tasks are instantiated for
+// both, so that the operator products stop after 2 P k-steps without a
+// branch per k-step.
+template <int MB>
+__device__ __forceinline__ bool full_groups(int c) {
+  return group_amps(c) == 2 * MB;
+}
+
 // one task of a beamsplitter pass: block s (c amplitudes from N = lo) on the
 // fibers f0 .. f0 + kBsChunk - 1
-template <int MB, int D>
+template <int MB, int P, int D>
 __device__ __forceinline__ void bs_task(const C* __restrict__ in, C* __restrict__ out, int64_t NF, const PairGeo& geo,
                                         int64_t sp, int64_t sq, const C* __restrict__ bb, int c, int lo, int s,
                                         int conj, int diff, int64_t f0) {
   constexpr int Q = Blk<MB>::Q;
-  const int P = group_amps(c);
   float A[MB][Blk<MB>::NS];
   load_op<MB>([&](int i, int k) { return op_entry(bb, c, conj ? 1 : 0, i, k); }, A, P);
   const int lane = threadIdx.x & 63;
@@ -236,7 +244,7 @@ __device__ __forceinline__ void bs_task(const C* __restrict__ in, C* __restrict_
       x[q] = (valid && q < P && a < c) ? v : mk<float>(0.f, 0.f);
     }
     f4 acc[MB];
-    apply_op<MB>(A, x, acc);
+    apply_op<MB>(A, x, acc, P);
     if (valid) {
 #pragma unroll
       for (int q = 0; q < Q; ++q)
@@ -258,28 +266,34 @@ k_bs_mfma(const C* __restrict__ in, C* __restrict__ out, int64_t NF, PairGeo geo
     const int c = bs_hi(s, D) - lo + 1;
     const C* bb = blk + bs_block_off(s, D);
     const int mb = (2 * c + 15) / 16;
-    if (mb == 1)
-      bs_task<1, D>(in, out, NF, geo, sp, sq, bb, c, lo, s, conj, diff, f0);
-    else if constexpr (D > 8) {
-      if (mb == 2)
-        bs_task<2, D>(in, out, NF, geo, sp, sq, bb, c, lo, s, conj, diff, f0);
-      else if constexpr (D > 16)
-        bs_task<3, D>(in, out, NF, geo, sp, sq, bb, c, lo, s, conj, diff, f0);
+#define SFAMD_BS_TASK(MB)                                                        \
+  if (full_groups<MB>(c))                                                        \
+    bs_task<MB, 2 * MB, D>(in, out, NF, geo, sp, sq, bb, c, lo, s, conj, diff, f0); \
+  else                                                                           \
+    bs_task<MB, 2 * MB - 1, D>(in, out, NF, geo, sp, sq, bb, c, lo, s, conj, diff, f0);
+    if (mb == 1) {
+      SFAMD_BS_TASK(1)
+    } else if constexpr (D > 8) {
+      if (mb == 2) {
+        SFAMD_BS_TASK(2)
+      } else if constexpr (D > 16) {
+        SFAMD_BS_TASK(3)
+      }
     }
+#undef SFAMD_BS_TASK
   }
 }
 
 // adjoint of one task: lam_in = B'^H lam_out; gradients through dB'/dp0
 // (table) and dB'/dp1 = i sg (wa n + wb N) B'[N][n] (sg = -1 on the conjugated
 // side).  opl: the workgroup's LDS for three operators of MB blocks.
-template <int MB, int D>
+template <int MB, int P, int D>
 __device__ __forceinline__ void bs_adj_task(const C* __restrict__ psi, const C* __restrict__ lo_, C* __restrict__ li,
                                             int64_t NF, const PairGeo& geo, int64_t sp, int64_t sq,
                                             const C* __restrict__ bb, const C* __restrict__ db, int c, int lo, int s,
                                             int conj, int wa, int wb, int diff, int64_t f0, float* opl, double& g0,
                                             double& g1) {
   constexpr int Q = Blk<MB>::Q, OPN = MB * Blk<MB>::NS * 64;
-  const int P = group_amps(c);
   __syncthreads();  // the previous task's operators are no longer read
   build_op_lds<MB>([&](int i, int k) { return op_entry(bb, c, conj ? 3 : 2, i, k); }, opl, P);
   build_op_lds<MB>([&](int i, int k) { return op_entry(db, c, conj ? 1 : 0, i, k); }, opl + OPN, P);
@@ -314,17 +328,17 @@ __device__ __forceinline__ void bs_adj_task(const C* __restrict__ psi, const C* 
       l[q] = in_range ? lv : mk<float>(0.f, 0.f);
     }
     f4 acc[MB];
-    apply_op_lds<MB>(opl, l, acc);
+    apply_op_lds<MB>(opl, l, acc, P);
     if (valid) {
 #pragma unroll
       for (int q = 0; q < Q; ++q)
         if (q < P && g * P + q < c) li[fb + (int64_t)(g * P + q) * step] = acc_at<MB>(acc, q);
     }
     float s0 = 0.f, s1 = 0.f;
-    apply_op_lds<MB>(opl + OPN, x, acc);
+    apply_op_lds<MB>(opl + OPN, x, acc, P);
 #pragma unroll
     for (int q = 0; q < Q; ++q) s0 += re_conj_mul(l[q], acc_at<MB>(acc, q));
-    apply_op_lds<MB>(opl + 2 * OPN, x, acc);
+    apply_op_lds<MB>(opl + 2 * OPN, x, acc, P);
 #pragma unroll
     for (int q = 0; q < Q; ++q) s1 += re_conj_mul(l[q], acc_at<MB>(acc, q));
     g0 += (double)s0;
@@ -348,17 +362,23 @@ k_bs_mfma_adj(const C* __restrict__ psi, const C* __restrict__ lo_, C* __restric
     const int c = bs_hi(s, D) - lo + 1;
     const int64_t off = bs_block_off(s, D);
     const int mb = (2 * c + 15) / 16;
-    if (mb == 1)
-      bs_adj_task<1, D>(psi, lo_, li, NF, geo, sp, sq, blk + off, dblk + off, c, lo, s, conj, wa, wb, diff, f0, opl, g0,
-                          g1);
-    else if constexpr (D > 8) {
-      if (mb == 2)
-        bs_adj_task<2, D>(psi, lo_, li, NF, geo, sp, sq, blk + off, dblk + off, c, lo, s, conj, wa, wb, diff, f0, opl, g0,
-                          g1);
-      else if constexpr (D > 16)
-        bs_adj_task<3, D>(psi, lo_, li, NF, geo, sp, sq, blk + off, dblk + off, c, lo, s, conj, wa, wb, diff, f0, opl, g0,
-                          g1);
+#define SFAMD_BS_ADJ_TASK(MB)                                                                                      \
+  if (full_groups<MB>(c))                                                                                          \
+    bs_adj_task<MB, 2 * MB, D>(psi, lo_, li, NF, geo, sp, sq, blk + off, dblk + off, c, lo, s, conj, wa, wb, diff, \
+                               f0, opl, g0, g1);                                                                   \
+  else                                                                                                             \
+    bs_adj_task<MB, 2 * MB - 1, D>(psi, lo_, li, NF, geo, sp, sq, blk + off, dblk + off, c, lo, s, conj, wa, wb,   \
+                                   diff, f0, opl, g0, g1);
+    if (mb == 1) {
+      SFAMD_BS_ADJ_TASK(1)
+    } else if constexpr (D > 8) {
+      if (mb == 2) {
+        SFAMD_BS_ADJ_TASK(2)
+      } else if constexpr (D > 16) {
+        SFAMD_BS_ADJ_TASK(3)
+      }
     }
+#undef SFAMD_BS_ADJ_TASK
   }
   mfma_write_partials(g0, g1, partial, nwg, zp1);
 }
@@ -433,9 +453,9 @@ k_dm_dense_mfma(const C* __restrict__ in, C* __restrict__ out, int64_t N, int64_
           }
           f4 acc[MB];
           if (pass == 0)
-            apply_op<MB>(AU, x, acc);
+            apply_op<MB>(AU, x, acc, P);
           else
-            apply_op<MB>(AC, x, acc);
+            apply_op<MB>(AC, x, acc, P);
           if (valid) {
 #pragma unroll
             for (int q = 0; q < P; ++q)
@@ -463,7 +483,7 @@ k_dm_dense_mfma(const C* __restrict__ in, C* __restrict__ out, int64_t N, int64_
           if (valid && g * P + q < D) x[q] = v;
         }
         f4 acc[MB];
-        apply_op<MB>(AU, x, acc);
+        apply_op<MB>(AU, x, acc, P);
 #pragma unroll
         for (int q = 0; q < P; ++q)
           if (g * P + q < D) sh[((g * P + q) * D + j) * 16 + c] = acc_at<MB>(acc, q);
@@ -480,7 +500,7 @@ k_dm_dense_mfma(const C* __restrict__ in, C* __restrict__ out, int64_t N, int64_
           if (g * P + q < D) x[q] = v;
         }
         f4 acc[MB];
-        apply_op<MB>(AC, x, acc);
+        apply_op<MB>(AC, x, acc, P);
         if (valid) {
 #pragma unroll
           for (int q = 0; q < P; ++q)

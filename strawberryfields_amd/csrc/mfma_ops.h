@@ -71,13 +71,17 @@ __device__ __forceinline__ C op_entry(const C* __restrict__ M, int n, int op, in
 // issued; the value is replaced by zero when the slot holds no amplitude)
 __device__ __forceinline__ int clamp_amp(int a, int n) { return a < n ? a : n - 1; }
 
-// acc = op x for this lane's 16-fiber block (x in group layout)
+// acc = op x for this lane's 16-fiber block (x in group layout).  Only the
+// first 2 P k-steps carry amplitudes (slots P .. Q - 1 are zero columns), so
+// the product stops there: P = group_amps(n).
 template <int MB>
-__device__ __forceinline__ void apply_op(const float (&A)[MB][Blk<MB>::NS], const C (&x)[Blk<MB>::Q], f4 (&acc)[MB]) {
+__device__ __forceinline__ void apply_op(const float (&A)[MB][Blk<MB>::NS], const C (&x)[Blk<MB>::Q], f4 (&acc)[MB],
+                                         int P) {
 #pragma unroll
   for (int mb = 0; mb < MB; ++mb) acc[mb] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int s = 0; s < Blk<MB>::NS; ++s) {
+    if (s >= 2 * P) continue;
     const float b = (s & 1) ? x[s >> 1].im : x[s >> 1].re;
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb) acc[mb] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[mb][s], b, acc[mb], 0, 0, 0);
@@ -118,12 +122,13 @@ __device__ __forceinline__ void build_op_lds(F entry, float* dst, int P) {
 }
 
 template <int MB>
-__device__ __forceinline__ void apply_op_lds(const float* opl, const C (&x)[Blk<MB>::Q], f4 (&acc)[MB]) {
+__device__ __forceinline__ void apply_op_lds(const float* opl, const C (&x)[Blk<MB>::Q], f4 (&acc)[MB], int P) {
   const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int mb = 0; mb < MB; ++mb) acc[mb] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int s = 0; s < Blk<MB>::NS; ++s) {
+    if (s >= 2 * P) continue;
     const float b = (s & 1) ? x[s >> 1].im : x[s >> 1].re;
 #pragma unroll
     for (int mb = 0; mb < MB; ++mb)

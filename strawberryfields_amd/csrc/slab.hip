@@ -548,7 +548,10 @@ __device__ __forceinline__ void slab_block(Cf* sh, const P& pol, int b, int n, c
     }
     pos0 += g * pa * step;
     f4 acc[MB];
-    sfamd::mf::apply_op<MB>(A, x, acc);
+    // every k-step: the block size is a runtime value here, and a runtime
+    // bound on the k-steps (a branch per k-step) measured slower than the
+    // products with the padding columns
+    sfamd::mf::apply_op<MB>(A, x, acc, Blk<MB>::Q);
     if (valid) {
 #pragma unroll
       for (int q = 0; q < Q; ++q)
@@ -619,15 +622,15 @@ __device__ __forceinline__ void slab_block_adj(const Cf* shx, Cf* shl, const P& 
     }
     pos0 += gq * step;
     f4 acc[MB];
-    sfamd::mf::apply_op<MB>(A0, x, acc);
+    sfamd::mf::apply_op<MB>(A0, x, acc, Blk<MB>::Q);
 #pragma unroll
     for (int q = 0; q < Q; ++q) s0 += re_conj_mul(l[q], sfamd::mf::acc_at<MB>(acc, q));
     if (P::kTwoParams) {
-      sfamd::mf::apply_op<MB>(A1, x, acc);
+      sfamd::mf::apply_op<MB>(A1, x, acc, Blk<MB>::Q);
 #pragma unroll
       for (int q = 0; q < Q; ++q) s1 += re_conj_mul(l[q], sfamd::mf::acc_at<MB>(acc, q));
     }
-    sfamd::mf::apply_op<MB>(AH, l, acc);
+    sfamd::mf::apply_op<MB>(AH, l, acc, Blk<MB>::Q);
     if (valid) {
 #pragma unroll
       for (int q = 0; q < Q; ++q)
