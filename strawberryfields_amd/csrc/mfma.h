@@ -30,4 +30,12 @@ int pair_bs_adj_mfma(const void* psi, const void* lam_out, void* lam_in, int64_t
 bool dm_dense_mfma_supported(int D, int dtype, int64_t sk);
 int dm_dense_mfma(const void* in, void* out, int64_t N, int D, int64_t sk, const void* M, hipStream_t s);
 
+// rho' = G rho G^H for a beamsplitter-family gate G on modes (m0, m1) of an
+// n-mode DM (axes [m: unconjugated, conjugated] per mode, mode 0 slowest), in
+// one pass; blocks as for pair_bs_mfma.  Supported when the modes after
+// max(m0, m1) leave an innermost run of >= 16 entries.
+bool dm_pair_mfma_supported(int n, int D, int m0, int m1, int dtype);
+int dm_pair_mfma(const void* in, void* out, int n, int D, int m0, int m1, const void* blocks, int diff,
+                 hipStream_t s);
+
 }  // namespace sfamd

@@ -512,6 +512,145 @@ k_dm_dense_mfma(const C* __restrict__ in, C* __restrict__ out, int64_t N, int64_
   }
 }
 
+// ------------------------------------------------------------------ two-sided two-mode gate on a DM
+// rho' = G rho G^H for a beamsplitter-family gate G on modes (m0, m1) of a DM
+// in one HBM pass (the reference applies the gate to one index pair, then its
+// conjugate to the other, in two passes: fockbackend/circuit.py:262-300,
+// 309-335).  G is block diagonal: block s on the unconjugated axis pair
+// (amplitudes k = 0 .. c_s - 1 of the block), block t on the conjugated pair
+// (k').  Over 16 consecutive positions tau of the innermost non-gate run, the
+// (s, t) part of rho is X[k][k'][tau], and rho'_(s,t) = G_s X G_t^H.  A
+// workgroup takes one (s, t) pair, loads its two operators into registers
+// once, and runs through a group of chunks of 16 positions: each chunk is
+// staged in LDS as [k][k'][tau] with batched loads (a fiber loop reading HBM
+// directly waits for each fiber's loads in turn), G_s is applied along k in
+// place (fibers (tau, k')), then conj(G_t) along k' (fibers (tau, u)) with the
+// results written to HBM.
+struct DmPairGeo {
+  int64_t ap, aq, bp, bq;  // strides of the first / second mode's axes: unconjugated (a), conjugated (b)
+  int64_t R, SM, SO;       // innermost non-gate run; strides of the middle and outer non-gate indices
+  int64_t M, nch, nchunk;  // middle extent; chunks of 16 per run; chunks in all
+};
+
+constexpr int kDmPairGroup = 8;  // chunks of 16 positions per task
+
+template <int MBS, int MBT, int D>
+__device__ __forceinline__ void dm_pair_task(const C* __restrict__ in, C* __restrict__ out, const DmPairGeo& geo,
+                                             const C* __restrict__ blk, int s, int t, int diff, int64_t ch0,
+                                             int64_t ch1, C* sh) {
+  constexpr int QS = Blk<MBS>::Q, QT = Blk<MBT>::Q;
+  const int lo_s = bs_lo(s, D), cs = bs_hi(s, D) - lo_s + 1;
+  const int lo_t = bs_lo(t, D), ct = bs_hi(t, D) - lo_t + 1;
+  const int ps = group_amps(cs), pt = group_amps(ct);
+  float AS[MBS][Blk<MBS>::NS], AT[MBT][Blk<MBT>::NS];
+  load_op<MBS>([&](int i, int k) { return op_entry(blk + bs_block_off(s, D), cs, 0, i, k); }, AS, ps);
+  load_op<MBT>([&](int i, int k) { return op_entry(blk + bs_block_off(t, D), ct, 1, i, k); }, AT, pt);
+  const int lane = threadIdx.x & 63;
+  const int cl = lane & 15, g = lane >> 4;
+  const int wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int64_t sa = block_step(geo.ap, geo.aq, diff), sb = block_step(geo.bp, geo.bq, diff);
+  const int64_t first = block_first(geo.ap, geo.aq, lo_s, s, D, diff) + block_first(geo.bp, geo.bq, lo_t, t, D, diff);
+  const int ne = cs * ct * 16;
+  for (int64_t ch = ch0; ch < ch1; ++ch) {
+    const int64_t om = ch / geo.nch;
+    const int64_t t0 = (ch - om * geo.nch) * 16;
+    const int64_t o = om / geo.M;
+    const int64_t base = o * geo.SO + (om - o * geo.M) * geo.SM + t0 + first;
+    // stage X[k][k'][tau] (LDS [k][k'][tau]) with kStageCopy loads in flight per
+    // thread; positions past the run read position t0 (their results are not stored)
+    for (int e0 = 0; e0 < ne; e0 += kStageCopy * (int)blockDim.x) {
+      C v[kStageCopy];
+#pragma unroll
+      for (int u = 0; u < kStageCopy; ++u) {
+        const int e = e0 + (int)threadIdx.x + u * (int)blockDim.x;
+        const int ec = e < ne ? e : ne - 1;
+        const int tau = ec & 15, kk = ec >> 4;
+        const int k = kk / ct, kp = kk - k * ct;
+        v[u] = in[base + (t0 + tau < geo.R ? tau : 0) + (int64_t)k * sa + (int64_t)kp * sb];
+      }
+#pragma unroll
+      for (int u = 0; u < kStageCopy; ++u) {
+        const int e = e0 + (int)threadIdx.x + u * (int)blockDim.x;
+        if (e < ne) sh[e] = v[u];
+      }
+    }
+    __syncthreads();
+    // in place: Y[u][k'][tau] = sum_k G_s[u][k] X[k][k'][tau]  (fibers (tau, k'))
+    for (int kb = wave; kb < ct; kb += nw) {
+      C x[QS];
+#pragma unroll
+      for (int q = 0; q < QS; ++q) {
+        const int a = g * ps + q;
+        const C v = sh[(clamp_amp(a, cs) * ct + kb) * 16 + cl];
+        x[q] = (q < ps && a < cs) ? v : mk<float>(0.f, 0.f);
+      }
+      f4 acc[MBS];
+      apply_op<MBS>(AS, x, acc, QS);
+#pragma unroll
+      for (int q = 0; q < QS; ++q)
+        if (q < ps && g * ps + q < cs) sh[((g * ps + q) * ct + kb) * 16 + cl] = acc_at<MBS>(acc, q);
+    }
+    __syncthreads();
+    // rho'[u][u'][tau] = sum_k' conj(G_t[u'][k']) Y[u][k'][tau]  (fibers (tau, u), to HBM)
+    const bool valid = t0 + cl < geo.R;
+    for (int u = wave; u < cs; u += nw) {
+      C x[QT];
+#pragma unroll
+      for (int q = 0; q < QT; ++q) {
+        const int a = g * pt + q;
+        const C v = sh[(u * ct + clamp_amp(a, ct)) * 16 + cl];
+        x[q] = (q < pt && a < ct) ? v : mk<float>(0.f, 0.f);
+      }
+      f4 acc[MBT];
+      apply_op<MBT>(AT, x, acc, QT);
+      if (valid) {
+#pragma unroll
+        for (int q = 0; q < QT; ++q)
+          if (q < pt && g * pt + q < ct)
+            out[base + cl + (int64_t)u * sa + (int64_t)(g * pt + q) * sb] = acc_at<MBT>(acc, q);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+template <int D>
+__global__ void __launch_bounds__(256, 3)
+k_dm_pair_mfma(const C* __restrict__ in, C* __restrict__ out, DmPairGeo geo, const C* __restrict__ blk, int diff) {
+  constexpr int NSB = 2 * D - 1;
+  __shared__ C sh[D * D * 16];
+  const int64_t ngrp = (geo.nchunk + kDmPairGroup - 1) / kDmPairGroup;
+  const int64_t ntask = (int64_t)NSB * NSB * ngrp;
+  for (int64_t task = blockIdx.x; task < ntask; task += gridDim.x) {
+    const int st = (int)(task % (NSB * NSB));
+    const int64_t ch0 = task / (NSB * NSB) * kDmPairGroup;
+    const int64_t ch1 = ch0 + kDmPairGroup < geo.nchunk ? ch0 + kDmPairGroup : geo.nchunk;
+    const int s = st / NSB, t = st - s * NSB;
+    const int mbs = (2 * (bs_hi(s, D) - bs_lo(s, D) + 1) + 15) / 16;
+    const int mbt = (2 * (bs_hi(t, D) - bs_lo(t, D) + 1) + 15) / 16;
+    // block sizes c <= D: mb <= Plan<D>::Mb, so only those variants exist
+#define SFAMD_DM_PAIR_T(MS)                                                  \
+  if (mbt == 1)                                                              \
+    dm_pair_task<MS, 1, D>(in, out, geo, blk, s, t, diff, ch0, ch1, sh);     \
+  else if constexpr (D > 8) {                                                \
+    if (mbt == 2)                                                            \
+      dm_pair_task<MS, 2, D>(in, out, geo, blk, s, t, diff, ch0, ch1, sh);   \
+    else if constexpr (D > 16)                                               \
+      dm_pair_task<MS, 3, D>(in, out, geo, blk, s, t, diff, ch0, ch1, sh);   \
+  }
+    if (mbs == 1) {
+      SFAMD_DM_PAIR_T(1)
+    } else if constexpr (D > 8) {
+      if (mbs == 2) {
+        SFAMD_DM_PAIR_T(2)
+      } else if constexpr (D > 16) {
+        SFAMD_DM_PAIR_T(3)
+      }
+    }
+#undef SFAMD_DM_PAIR_T
+  }
+}
+
 #define SFAMD_MFMA_D(X) \
   X(2) X(3) X(4) X(5) X(6) X(7) X(8) X(9) X(10) X(11) X(12) X(13) X(14) X(15) X(16) X(17) X(18) X(19) X(20)
 
@@ -623,6 +762,52 @@ int dm_dense_mfma(const void* in, void* out, int64_t N, int D, int64_t sk, const
     else                                                                                                     \
       hipLaunchKernelGGL((k_dm_dense_mfma<d, false>), dim3(grid), dim3(kBlock), 0, s, (const C*)in,         \
                          (C*)out, N, sk, (const C*)M);                                                       \
+    break;
+    SFAMD_MFMA_D(CASE)
+#undef CASE
+    default:
+      return SFAMD_E_CUTOFF;
+  }
+  return (int)hipGetLastError();
+}
+
+bool dm_pair_mfma_supported(int n, int D, int m0, int m1, int dtype) {
+  const int q = m0 > m1 ? m0 : m1;
+  if (!mfma_supported(D, dtype) || m0 == m1 || q >= n - 1) return false;
+  int64_t R = 1;
+  for (int i = 0; i < 2 * (n - 1 - q); ++i) R *= D;
+  return R >= 16;
+}
+
+int dm_pair_mfma(const void* in, void* out, int n, int D, int m0, int m1, const void* blocks, int diff,
+                 hipStream_t s) {
+  if (!dm_pair_mfma_supported(n, D, m0, m1, SFAMD_C64)) return SFAMD_E_ARG;
+  auto pw = [D](int e) {
+    int64_t r = 1;
+    for (int i = 0; i < e; ++i) r *= D;
+    return r;
+  };
+  const int p = m0 < m1 ? m0 : m1, q = m0 > m1 ? m0 : m1;
+  DmPairGeo geo;
+  // mode m: unconjugated axis stride D^(2(n-1-m)+1), conjugated D^(2(n-1-m))
+  geo.ap = pw(2 * (n - 1 - m0) + 1);
+  geo.aq = pw(2 * (n - 1 - m1) + 1);
+  geo.bp = pw(2 * (n - 1 - m0));
+  geo.bq = pw(2 * (n - 1 - m1));
+  geo.R = pw(2 * (n - 1 - q));
+  geo.SM = pw(2 * (n - q));
+  geo.SO = pw(2 * (n - p));
+  geo.M = pw(2 * (q - p - 1));
+  geo.nch = (geo.R + 15) / 16;
+  geo.nchunk = pw(2 * p) * geo.M * geo.nch;
+  const int64_t nsb = 2 * D - 1;
+  const int64_t ntask = nsb * nsb * ((geo.nchunk + kDmPairGroup - 1) / kDmPairGroup);
+  const int grid = (int)(ntask < kMaxGrid ? ntask : kMaxGrid);
+  switch (D) {
+#define CASE(d)                                                                                          \
+  case d:                                                                                                \
+    hipLaunchKernelGGL((k_dm_pair_mfma<d>), dim3(grid), dim3(kBlock), 0, s, (const C*)in, (C*)out, geo, \
+                       (const C*)blocks, diff);                                                          \
     break;
     SFAMD_MFMA_D(CASE)
 #undef CASE

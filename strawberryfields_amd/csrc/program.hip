@@ -199,6 +199,9 @@ int apply_op(const OpPlan& op, int n, int D, const void* table, const void* in, 
     case SFAMD_OP_MZGATE:
     case SFAMD_OP_CROSS_KERR:
       if (op.mixed_in) {
+        // one pass (rho' = G rho G^H) when a run of inner modes follows the pair
+        if (dm_pair_mfma_supported(n, D, op.m0, op.m1, dtype))
+          return dm_pair_mfma(in, out, n, D, op.m0, op.m1, g, op.kind == SFAMD_OP_TWO_MODE_SQUEEZE, s);
         if (!tmp) return SFAMD_E_WORKSPACE;
         RET_IF(pair_pass(op.kind, in, tmp, NM, D, bra_stride_dm(n, D, op.m0), bra_stride_dm(n, D, op.m1), g, 0,
                          dtype, s));
