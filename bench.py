@@ -348,7 +348,7 @@ def measure_c4(steps):
     circs = c4["circuits"]
     tuner = runtime.Autotuner(path=os.environ.get("SFAMD_AUTOTUNE_CACHE"))
     t0 = time.perf_counter()
-    be = runtime.BatchEvaluator(circs, torch.complex64, dev, nstreams=8, tuner=tuner)
+    be = runtime.BatchEvaluator(circs, torch.complex64, dev, tuner=tuner)
     t_setup = time.perf_counter() - t0
     plist = []
     for c in circs:

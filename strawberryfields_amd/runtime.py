@@ -810,11 +810,14 @@ class BatchEvaluator:
     (every step input, cotangents, gradients).  The capture forks the circuits
     over `nstreams` streams (independent graph branches the GPU runs
     concurrently, so small circuits fill the compute units left idle by large
-    ones) and joins them before the graph ends.  A call copies the stacked
-    parameters into the static buffers and launches the graph once.
+    ones) and joins them before the graph ends.  Most circuits of a mixed batch
+    are small and bound by per-launch latency, so many branches pay: on the
+    256-circuit workload 8 streams give 5.3 batches/s, 32 give 9.5 and 64 give
+    9.8.  A call copies the stacked parameters into the static buffers and
+    launches the graph once.
     """
 
-    def __init__(self, circuits, dtype=torch.complex64, device="cuda", nstreams=8, tuner=None, use_graph=True):
+    def __init__(self, circuits, dtype=torch.complex64, device="cuda", nstreams=64, tuner=None, use_graph=True):
         lib = _lib.load()
         dev = torch.device(device)
         self.device = dev
@@ -847,8 +850,15 @@ class BatchEvaluator:
         self.losses = torch.zeros(len(self.items), dtype=torch.float64, device=dev)
         self.main = torch.cuda.Stream(device=dev)
         self.streams = [torch.cuda.Stream(device=dev) for _ in range(nstreams)]
-        # largest circuits first, dealt round-robin so branches carry similar bytes
-        self.order = sorted(range(len(self.items)), key=lambda i: -self.items[i]["psi"].numel())
+        # largest circuits first, each to the branch with the least work so far,
+        # so that the branches end together
+        self.order = sorted(range(len(self.items)), key=lambda i: -self._work(self.items[i]))
+        load = [0.0] * nstreams
+        self.assign = [[] for _ in range(nstreams)]
+        for k in self.order:
+            si = min(range(nstreams), key=lambda j: load[j])
+            self.assign[si].append(k)
+            load[si] += self._work(self.items[k])
         self.graph = None
         torch.cuda.synchronize(dev)
         self._enqueue_all()
@@ -863,6 +873,12 @@ class BatchEvaluator:
                 _lib.call("sfamd_graph_end", sp, ctypes.byref(g))
             self.graph = g
         torch.cuda.synchronize(dev)
+
+    @staticmethod
+    def _work(it):
+        # per step: a fixed launch and latency cost (about that of streaming
+        # 2^18 amplitudes) plus the state's size
+        return max(1, it["cc"].nsteps) * (1.0 + it["psi"].numel() / float(1 << 18))
 
     def _enqueue_one(self, it, k):
         cc = it["cc"]
@@ -888,8 +904,7 @@ class BatchEvaluator:
         for si, s in enumerate(self.streams):
             s.wait_event(fork)
             with torch.cuda.stream(s):
-                for j in range(si, len(self.order), len(self.streams)):
-                    k = self.order[j]
+                for k in self.assign[si]:
                     self._enqueue_one(self.items[k], k)
             ev = torch.cuda.Event()
             ev.record(s)
