@@ -62,10 +62,51 @@ __device__ __forceinline__ int64_t fiber_base(int64_t f, int64_t R, int D) {
   return o * D * R + (f - o * R);
 }
 
+// Innermost axis (INNER: stride 1, a whole number of 16-fiber chains): the
+// 16 fibers of a chain are 16 D contiguous amplitudes, but a lane's slots of
+// one fiber are D amplitudes away from the next lane's, so each 8-byte load
+// instruction would touch 16 cache lines.  Instead each wave copies its chain
+// through its own LDS buffer with coalesced 16-byte accesses (64 lanes read
+// 1 KB contiguous per instruction) and the lanes take their slots from there;
+// results go back the same way.  LDS accesses of one wave complete in order,
+// so the buffer needs no barrier, only the compiler's ordering.
 template <int D>
+__device__ __forceinline__ void chain_in(const C* __restrict__ src, C* buf) {
+  constexpr int NV = 8 * D, IT = (NV + 63) / 64;  // 16-byte vectors of 16 D amplitudes
+  const int lane = threadIdx.x & 63;
+  f4 v[IT];
+#pragma unroll
+  for (int i = 0; i < IT; ++i) {
+    const int j = lane + 64 * i;
+    v[i] = reinterpret_cast<const f4*>(src)[j < NV ? j : NV - 1];
+  }
+#pragma unroll
+  for (int i = 0; i < IT; ++i) {
+    const int j = lane + 64 * i;
+    if (j < NV) reinterpret_cast<f4*>(buf)[j] = v[i];
+  }
+  __asm__ volatile("" ::: "memory");
+}
+
+template <int D>
+__device__ __forceinline__ void chain_out(const C* buf, C* __restrict__ dst) {
+  constexpr int NV = 8 * D, IT = (NV + 63) / 64;
+  const int lane = threadIdx.x & 63;
+  __asm__ volatile("" ::: "memory");
+#pragma unroll
+  for (int i = 0; i < IT; ++i) {
+    const int j = lane + 64 * i;
+    if (j < NV) reinterpret_cast<f4*>(dst)[j] = reinterpret_cast<const f4*>(buf)[j];
+  }
+  __asm__ volatile("" ::: "memory");
+}
+
+template <int D, bool INNER>
 __global__ void __launch_bounds__(256)
 k_axis_mfma(const C* __restrict__ in, C* __restrict__ out, int64_t NF, int64_t R, const C* __restrict__ M, int op) {
-  constexpr int Q = Plan<D>::Q, P = Plan<D>::P;
+  constexpr int Q = Plan<D>::Q, P = Plan<D>::P, CH = 16 * D;
+  __shared__ f4 stage_v[INNER ? 4 * CH / 2 : 1];
+  C* buf = reinterpret_cast<C*>(stage_v) + (INNER ? (threadIdx.x >> 6) * CH : 0);
   float A[Plan<D>::Mb][Plan<D>::NS];
   load_operator<D>(M, op, A);
   const int lane = threadIdx.x & 63;
@@ -77,17 +118,23 @@ k_axis_mfma(const C* __restrict__ in, C* __restrict__ out, int64_t NF, int64_t R
     const bool valid = f < NF;
     const int64_t fb = fiber_base(valid ? f : 0, R, D);
     C x[Q];
+    if constexpr (INNER) chain_in<D>(in + blk * CH, buf);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       x[q] = mk<float>(0.f, 0.f);
       if (q >= P) continue;
       const int a = g * P + q;
-      const C v = in[fb + (int64_t)clamp_amp(a, D) * R];
+      const C v = INNER ? buf[c * D + clamp_amp(a, D)] : in[fb + (int64_t)clamp_amp(a, D) * R];
       if (valid && a < D) x[q] = v;
     }
     f4 acc[Plan<D>::Mb];
     apply_operator<D>(A, x, acc);
-    if (valid) {
+    if constexpr (INNER) {
+#pragma unroll
+      for (int q = 0; q < P; ++q)
+        if (g * P + q < D) buf[c * D + g * P + q] = acc_amp<D>(acc, q);
+      chain_out<D>(buf, out + blk * CH);
+    } else if (valid) {
 #pragma unroll
       for (int q = 0; q < P; ++q)
         if (g * P + q < D) out[fb + (int64_t)(g * P + q) * R] = acc_amp<D>(acc, q);
@@ -96,13 +143,16 @@ k_axis_mfma(const C* __restrict__ in, C* __restrict__ out, int64_t NF, int64_t R
 }
 
 // adjoint: lam_in = U'^H lam_out;  g_p += Re sum_i conj(lam_out[i]) (dU'_p x)[i]
-template <int D>
+template <int D, bool INNER>
 __global__ void __launch_bounds__(256)
 k_axis_mfma_adj(const C* __restrict__ psi, const C* __restrict__ lo, C* __restrict__ li, int64_t NF, int64_t R,
                 const C* __restrict__ M, const C* __restrict__ dM0, const C* __restrict__ dM1, int conj,
                 double* __restrict__ partial, int nwg, int zp1) {
-  constexpr int MB = Plan<D>::Mb, Q = Plan<D>::Q, P = Plan<D>::P, OPN = MB * Plan<D>::NS * 64;
+  constexpr int MB = Plan<D>::Mb, Q = Plan<D>::Q, P = Plan<D>::P, OPN = MB * Plan<D>::NS * 64, CH = 16 * D;
   __shared__ float opl[3 * OPN];
+  __shared__ f4 stage_v[INNER ? 2 * 4 * CH / 2 : 1];  // per wave: the chain of psi, then of lam_out
+  C* bx = reinterpret_cast<C*>(stage_v) + (INNER ? 2 * (threadIdx.x >> 6) * CH : 0);
+  C* bl = bx + (INNER ? CH : 0);
   build_op_lds<MB>([&](int i, int k) { return op_entry(M, D, conj ? 3 : 2, i, k); }, opl, P);
   build_op_lds<MB>([&](int i, int k) { return op_entry(dM0, D, conj ? 1 : 0, i, k); }, opl + OPN, P);
   build_op_lds<MB>([&](int i, int k) { return op_entry(dM1, D, conj ? 1 : 0, i, k); }, opl + 2 * OPN, P);
@@ -120,13 +170,18 @@ k_axis_mfma_adj(const C* __restrict__ psi, const C* __restrict__ lo, C* __restri
     const bool valid = f < NF;
     const int64_t fb = fiber_base(valid ? f : 0, R, D);
     C x[Q], l[Q];
+    if constexpr (INNER) {
+      chain_in<D>(psi + blk * CH, bx);
+      chain_in<D>(lo + blk * CH, bl);
+    }
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
       x[q] = l[q] = mk<float>(0.f, 0.f);
       if (q >= P) continue;
       const int a = g * P + q;
       const int64_t o = fb + (int64_t)clamp_amp(a, D) * R;
-      const C xv = psi[o], lv = lo[o];
+      const C xv = INNER ? bx[c * D + clamp_amp(a, D)] : psi[o];
+      const C lv = INNER ? bl[c * D + clamp_amp(a, D)] : lo[o];
       if (valid && a < D) {
         x[q] = xv;
         l[q] = lv;
@@ -134,7 +189,12 @@ k_axis_mfma_adj(const C* __restrict__ psi, const C* __restrict__ lo, C* __restri
     }
     f4 acc[MB];
     apply_op_lds<MB>(opl, l, acc, P);
-    if (valid) {
+    if constexpr (INNER) {
+#pragma unroll
+      for (int q = 0; q < P; ++q)
+        if (g * P + q < D) bl[c * D + g * P + q] = acc_at<MB>(acc, q);
+      chain_out<D>(bl, li + blk * CH);
+    } else if (valid) {
 #pragma unroll
       for (int q = 0; q < P; ++q)
         if (g * P + q < D) li[fb + (int64_t)(g * P + q) * R] = acc_at<MB>(acc, q);
@@ -663,11 +723,16 @@ bool mfma_supported(int D, int dtype) { return dtype == SFAMD_C64 && D >= 2 && D
 int axis_dense_mfma(const void* in, void* out, int64_t N, int D, int64_t R, const void* M, int conj, hipStream_t s) {
   const int64_t NF = N / D;
   const int grid = grid_for((NF + 15) / 16 * 64);
+  const bool inner = R == 1 && NF % 16 == 0;  // whole 16-fiber chains, contiguous
   switch (D) {
 #define CASE(d)                                                                                               \
   case d:                                                                                                     \
-    hipLaunchKernelGGL((k_axis_mfma<d>), dim3(grid), dim3(kBlock), 0, s, (const C*)in, (C*)out, NF, R,       \
-                       (const C*)M, conj ? 1 : 0);                                                            \
+    if (inner)                                                                                                \
+      hipLaunchKernelGGL((k_axis_mfma<d, true>), dim3(grid), dim3(kBlock), 0, s, (const C*)in, (C*)out, NF, R, \
+                         (const C*)M, conj ? 1 : 0);                                                          \
+    else                                                                                                      \
+      hipLaunchKernelGGL((k_axis_mfma<d, false>), dim3(grid), dim3(kBlock), 0, s, (const C*)in, (C*)out, NF,   \
+                         R, (const C*)M, conj ? 1 : 0);                                                       \
     break;
     SFAMD_MFMA_D(CASE)
 #undef CASE
@@ -681,11 +746,18 @@ int axis_dense_adj_mfma(const void* psi, const void* lam_out, void* lam_in, int6
                         const void* M, const void* dM0, const void* dM1, int conj, double* partial, int nwg, int zp1,
                         hipStream_t s) {
   const int64_t NF = N / D;
+  const bool inner = R == 1 && NF % 16 == 0;  // whole 16-fiber chains, contiguous
   switch (D) {
 #define CASE(d)                                                                                                    \
   case d:                                                                                                          \
-    hipLaunchKernelGGL((k_axis_mfma_adj<d>), dim3(nwg), dim3(kBlock), 0, s, (const C*)psi, (const C*)lam_out,     \
-                       (C*)lam_in, NF, R, (const C*)M, (const C*)dM0, (const C*)dM1, conj, partial, nwg, zp1);    \
+    if (inner)                                                                                                     \
+      hipLaunchKernelGGL((k_axis_mfma_adj<d, true>), dim3(nwg), dim3(kBlock), 0, s, (const C*)psi,                \
+                         (const C*)lam_out, (C*)lam_in, NF, R, (const C*)M, (const C*)dM0, (const C*)dM1, conj,  \
+                         partial, nwg, zp1);                                                                     \
+    else                                                                                                           \
+      hipLaunchKernelGGL((k_axis_mfma_adj<d, false>), dim3(nwg), dim3(kBlock), 0, s, (const C*)psi,               \
+                         (const C*)lam_out, (C*)lam_in, NF, R, (const C*)M, (const C*)dM0, (const C*)dM1, conj,  \
+                         partial, nwg, zp1);                                                                     \
     break;
     SFAMD_MFMA_D(CASE)
 #undef CASE
