@@ -802,6 +802,20 @@ class Autotuner:
 
 
 # --------------------------------------------------------------------- batches
+def deal_by_work(works, nbranches):
+    """Items dealt to `nbranches` branches so that the branches end together:
+    largest first, each to the branch with the least work so far (ties to the
+    lowest branch).  Returns the item indices of each branch, in dealing order."""
+    order = sorted(range(len(works)), key=lambda i: (-works[i], i))
+    load = [0.0] * nbranches
+    out = [[] for _ in range(nbranches)]
+    for k in order:
+        b = min(range(nbranches), key=lambda j: (load[j], j))
+        out[b].append(k)
+        load[b] += works[k]
+    return out
+
+
 class BatchEvaluator:
     """Gradient of the total mean photon number of many independent pure-state
     circuits, replayed from ONE captured HIP graph.
@@ -850,15 +864,7 @@ class BatchEvaluator:
         self.losses = torch.zeros(len(self.items), dtype=torch.float64, device=dev)
         self.main = torch.cuda.Stream(device=dev)
         self.streams = [torch.cuda.Stream(device=dev) for _ in range(nstreams)]
-        # largest circuits first, each to the branch with the least work so far,
-        # so that the branches end together
-        self.order = sorted(range(len(self.items)), key=lambda i: -self._work(self.items[i]))
-        load = [0.0] * nstreams
-        self.assign = [[] for _ in range(nstreams)]
-        for k in self.order:
-            si = min(range(nstreams), key=lambda j: load[j])
-            self.assign[si].append(k)
-            load[si] += self._work(self.items[k])
+        self.assign = deal_by_work([self._work(it) for it in self.items], nstreams)
         self.graph = None
         torch.cuda.synchronize(dev)
         self._enqueue_all()

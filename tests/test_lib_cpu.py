@@ -607,3 +607,25 @@ def test_planner_handles_gates_too_large_for_the_occupancy_budget():
                 seen += step
             assert sorted(seen) == list(range(len(ops))), (n, D, fuse)
             assert lib.sfamd_prog_destroy(h) == 0
+
+
+def test_batch_branches_are_dealt_by_work():
+    """The batch evaluator's dealing of circuits over graph branches: every
+    circuit exactly once, deterministic, and largest-first to the least-loaded
+    branch, so the heaviest branch exceeds the lightest by at most the largest
+    item (the greedy bound; a round-robin deal of the same heavy-tailed sizes
+    breaks it)."""
+    from strawberryfields_amd import runtime
+
+    rng = np.random.default_rng(5)
+    works = list(rng.pareto(1.5, size=256) * 10 + 1)
+    for nb in (1, 3, 8, 64, 300):
+        br = runtime.deal_by_work(works, nb)
+        assert len(br) == nb
+        assert sorted(k for b in br for k in b) == list(range(len(works)))
+        loads = [sum(works[k] for k in b) for b in br]
+        assert max(loads) - min(loads) <= max(works) + 1e-9, nb
+        assert br == runtime.deal_by_work(works, nb)
+    rr = [[k for k in range(len(works)) if k % 8 == b] for b in range(8)]
+    rr_loads = [sum(works[k] for k in b) for b in rr]
+    assert max(rr_loads) - min(rr_loads) > max(works)
