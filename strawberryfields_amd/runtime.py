@@ -369,6 +369,43 @@ def reduced_dm(state, n, D, mixed, modes):
     return out
 
 
+def _reduced_dm_torch(state, n, D, mixed, modes):
+    """The reduced DM as torch contractions on the device (differentiable):
+    rho_red = M M^H with M the ket's kept modes x the rest, or the trace of
+    the DM over the rest modes' (bra, ket) pairs (states.py:613-642)."""
+    rest = [m for m in range(n) if m not in modes]
+    k = len(modes)
+    if mixed:
+        perm = [a for m in modes for a in (2 * m, 2 * m + 1)] + [2 * r for r in rest] + [2 * r + 1 for r in rest]
+        x = state.permute(perm).reshape(D ** (2 * k), D ** (n - k), D ** (n - k))
+        return x.diagonal(dim1=1, dim2=2).sum(-1).reshape([D] * (2 * k))
+    m = state.permute(list(modes) + rest).reshape(D**k, D ** (n - k))
+    r = (m @ m.conj().transpose(0, 1).contiguous()).reshape([D] * (2 * k))
+    return r.permute([a for i in range(k) for a in (i, k + i)])
+
+
+class ReducedDM(torch.autograd.Function):
+    """Reduced DM of sorted `modes` with a gradient: the forward is the
+    library's reduction kernel; the backward differentiates the same
+    contraction written as torch matrix products on the device (the TF
+    backend's reduced_dm is differentiable, tfbackend/states.py:307)."""
+
+    @staticmethod
+    def forward(ctx, state, n, D, mixed, modes):
+        ctx.save_for_backward(state)
+        ctx.meta = (n, D, mixed, list(modes))
+        return reduced_dm(state.detach(), n, D, mixed, list(modes))
+
+    @staticmethod
+    def backward(ctx, g):
+        (state,) = ctx.saved_tensors
+        n, D, mixed, modes = ctx.meta
+        with torch.enable_grad():
+            s = state.detach().requires_grad_(True)
+            (gs,) = torch.autograd.grad(_reduced_dm_torch(s, n, D, mixed, modes), s, g)
+        return gs, None, None, None, None
+
+
 def diagonal_expectation(state, n, D, mixed, modes, values):
     """sum_I p(I) prod_{m in modes} values[I_m] (FP64); returns a 0-dim float64 tensor."""
     lib = _lib.load()

@@ -132,12 +132,10 @@ class FockStateHIP:
             raise ValueError(
                 "The number of specified modes cannot be larger than the number of subsystems."
             )
-        if self._data.requires_grad:
-            raise NotImplementedError(
-                "reduced_dm is not differentiable; use all_fock_probs or mean_photon for gradients"
-            )
         if modes == list(range(self._modes)):
             return self.dm()
+        if self._data.requires_grad:
+            return runtime.ReducedDM.apply(self._data, self._modes, self._cutoff, not self._pure, modes)
         return runtime.reduced_dm(self._data, self._modes, self._cutoff, not self._pure, modes)
 
     def diagonal_expectation(self, modes, values):

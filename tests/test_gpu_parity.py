@@ -359,6 +359,43 @@ def _fd_grad(ops, n, D, pure, target, weights, h=1e-5):
 
 
 @pytest.mark.parametrize("dtype", [C128, C64])
+@pytest.mark.parametrize("pure", [True, False])
+def test_reduced_dm_gradient_vs_finite_differences(dtype, pure):
+    """reduced_dm is differentiable, as the TF backend's is
+    (tfbackend/states.py:307): the gradient of a real cost Re sum conj(W) rho_02
+    of the reduced DM of modes (0, 2), W a seeded complex array, through the
+    circuit's adjoint and the reduced DM's backward, against central
+    differences of the complex128 oracle; the forward value against the
+    oracle's reduced DM."""
+    n, D = 3, 4
+    rng = np.random.default_rng(77 + int(pure))
+    ops = circuits.layered(n, 1, rng, sd=0.3, loss=not pure)
+    W = rng.normal(size=[D] * 4) + 1j * rng.normal(size=[D] * 4)
+
+    def cost(flat):
+        k, new = 0, []
+        for name, modes, p in ops:
+            new.append((name, modes, tuple(flat[k : k + len(p)])))
+            k += len(p)
+        st, rp = fo.run_ops(new, n, D, pure=pure)
+        return float(np.real(np.sum(np.conj(W) * fo.reduced_dm(st, n, rp, [0, 2]))))
+
+    flat = np.array([v for _, _, p in ops for v in p], dtype=float)
+    h = 1e-5
+    g_fd = np.array([(cost(flat + h * e) - cost(flat - h * e)) / (2 * h) for e in np.eye(len(flat))])
+    params = torch.tensor(flat, dtype=torch.float64, requires_grad=True)
+    st = _run_params(ops, n, D, dtype, pure, params)
+    red = st.reduced_dm([0, 2])
+    st_ref, rp = fo.run_ops(ops, n, D, pure=pure)
+    assert np.abs(to_np(red) - fo.reduced_dm(st_ref, n, rp, [0, 2])).max() < TOL[dtype]
+    Wt = torch.as_tensor(W, dtype=dtype, device=DEV)
+    (Wt.conj() * red).sum().real.to(torch.float64).backward()
+    g = params.grad.numpy()
+    rel = np.linalg.norm(g - g_fd) / np.linalg.norm(g_fd)
+    assert rel < (1e-8 if dtype == C128 else 1e-5), rel
+
+
+@pytest.mark.parametrize("dtype", [C128, C64])
 def test_gradient_pure_layer_vs_finite_differences(dtype):
     n, D = 3, 6
     rng = np.random.default_rng(21)
