@@ -531,7 +531,10 @@ __global__ void __launch_bounds__(sfamd::stage_threads(D, K)) k_stage_adj(const 
       const sfamd::StageGate& g = a.g[gi];
       C<T>* dst = tiles + (gi + 1) * Gm::DK;
       gate_fwd<T, D, K>(g, me, mats + g.amoff, tiles + gi * Gm::DK, [dst](int, int e, C<T> v) { dst[e] = v; });
-      __syncthreads();
+      // the last gate's output is read only by the adjoint of that gate, at
+      // each thread's own elements (gate_fwd writes, and gate_adj reads pin and
+      // pout, at the thread's own e), so it needs no barrier
+      if (gi + 1 < a.ngates) __syncthreads();
     }
     C<T>* lo_ = lamA;
     C<T>* li = lamB;
