@@ -525,11 +525,14 @@ struct LossPol {
   __device__ Cf dentry1(int, int, int, int) const { return mk<float>(0.f, 0.f); }
 };
 
-template <int MB, int D, class P>
+// PA = group_amps(n), the amplitudes per lane group, is Q - 1 or Q for every
+// block size n; the blocks are instantiated for both so that the operator
+// products stop after 2 PA k-steps without a branch per k-step.
+template <int MB, int PA, int D, class P>
 __device__ __forceinline__ void slab_block(Cf* sh, const P& pol, int b, int n, const Tile& t, const View& v) {
   constexpr int Q = Blk<MB>::Q;
   float A[MB][Blk<MB>::NS];
-  const int pa = sfamd::mf::group_amps(n);  // amplitudes per lane group
+  constexpr int pa = PA;
   sfamd::mf::load_op<MB>([&](int i, int k) { return pol.entry(b, n, i, k); }, A, pa);
   const int per = pol.per_block(t, v);
   const int lane = threadIdx.x & 63;
@@ -548,10 +551,7 @@ __device__ __forceinline__ void slab_block(Cf* sh, const P& pol, int b, int n, c
     }
     pos0 += g * pa * step;
     f4 acc[MB];
-    // every k-step: the block size is a runtime value here, and a runtime
-    // bound on the k-steps (a branch per k-step) measured slower than the
-    // products with the padding columns
-    sfamd::mf::apply_op<MB>(A, x, acc, Blk<MB>::Q);
+    sfamd::mf::apply_op<MB>(A, x, acc, pa);
     if (valid) {
 #pragma unroll
       for (int q = 0; q < Q; ++q)
@@ -573,14 +573,21 @@ __global__ void __launch_bounds__(256) k_slab_mfma(const Cf* __restrict__ in, Cf
     for (int b = wave; b < P::NB; b += nw) {
       const int n = pol.size(b);
       const int mb = (2 * n + 15) / 16;
-      if (mb == 1)
-        slab_block<1, D>(sh, pol, b, n, t, v);
-      else if constexpr (D > 8) {
-        if (mb == 2)
-          slab_block<2, D>(sh, pol, b, n, t, v);
-        else if constexpr (D > 16)
-          slab_block<3, D>(sh, pol, b, n, t, v);
+#define SFAMD_SLAB_BLOCK(MB)                            \
+  if (sfamd::mf::group_amps(n) == 2 * MB)               \
+    slab_block<MB, 2 * MB, D>(sh, pol, b, n, t, v);     \
+  else                                                  \
+    slab_block<MB, 2 * MB - 1, D>(sh, pol, b, n, t, v);
+      if (mb == 1) {
+        SFAMD_SLAB_BLOCK(1)
+      } else if constexpr (D > 8) {
+        if (mb == 2) {
+          SFAMD_SLAB_BLOCK(2)
+        } else if constexpr (D > 16) {
+          SFAMD_SLAB_BLOCK(3)
+        }
       }
+#undef SFAMD_SLAB_BLOCK
     }
     __syncthreads();
     tile_store(sh, out, t, v, D);
@@ -593,12 +600,12 @@ __global__ void __launch_bounds__(256) k_slab_mfma(const Cf* __restrict__ in, Cf
 // then lam_in = B^H lam overwrites lam in place (each fiber belongs to this wave
 // alone).  The three operators live in registers: the kernel runs one 512-thread
 // workgroup per CU (its LDS), so registers are not what bounds occupancy.
-template <int MB, int D, class P>
+template <int MB, int PA, int D, class P>
 __device__ __forceinline__ void slab_block_adj(const Cf* shx, Cf* shl, const P& pol, int b, int n, const Tile& t,
                                                const View& v, double& g0, double& g1) {
   constexpr int Q = Blk<MB>::Q;
   float A0[MB][Blk<MB>::NS], A1[MB][Blk<MB>::NS], AH[MB][Blk<MB>::NS];
-  const int pa = sfamd::mf::group_amps(n);  // amplitudes per lane group
+  constexpr int pa = PA;
   sfamd::mf::load_op<MB>([&](int i, int k) { return pol.dentry0(b, n, i, k); }, A0, pa);
   if (P::kTwoParams) sfamd::mf::load_op<MB>([&](int i, int k) { return pol.dentry1(b, n, i, k); }, A1, pa);
   sfamd::mf::load_op<MB>([&](int i, int k) { return pol.entry_h(b, n, i, k); }, AH, pa);
@@ -622,15 +629,15 @@ __device__ __forceinline__ void slab_block_adj(const Cf* shx, Cf* shl, const P& 
     }
     pos0 += gq * step;
     f4 acc[MB];
-    sfamd::mf::apply_op<MB>(A0, x, acc, Blk<MB>::Q);
+    sfamd::mf::apply_op<MB>(A0, x, acc, pa);
 #pragma unroll
     for (int q = 0; q < Q; ++q) s0 += re_conj_mul(l[q], sfamd::mf::acc_at<MB>(acc, q));
     if (P::kTwoParams) {
-      sfamd::mf::apply_op<MB>(A1, x, acc, Blk<MB>::Q);
+      sfamd::mf::apply_op<MB>(A1, x, acc, pa);
 #pragma unroll
       for (int q = 0; q < Q; ++q) s1 += re_conj_mul(l[q], sfamd::mf::acc_at<MB>(acc, q));
     }
-    sfamd::mf::apply_op<MB>(AH, l, acc, Blk<MB>::Q);
+    sfamd::mf::apply_op<MB>(AH, l, acc, pa);
     if (valid) {
 #pragma unroll
       for (int q = 0; q < Q; ++q)
@@ -659,14 +666,21 @@ k_slab_mfma_adj(const Cf* __restrict__ psi, const Cf* __restrict__ lo_, Cf* __re
     for (int b = wave; b < P::NB; b += nw) {
       const int n = pol.size(b);
       const int mb = (2 * n + 15) / 16;
-      if (mb == 1)
-        slab_block_adj<1, D>(shx, shl, pol, b, n, t, v, g0, g1);
-      else if constexpr (D > 8) {
-        if (mb == 2)
-          slab_block_adj<2, D>(shx, shl, pol, b, n, t, v, g0, g1);
-        else if constexpr (D > 16)
-          slab_block_adj<3, D>(shx, shl, pol, b, n, t, v, g0, g1);
+#define SFAMD_SLAB_BLOCK_ADJ(MB)                                        \
+  if (sfamd::mf::group_amps(n) == 2 * MB)                               \
+    slab_block_adj<MB, 2 * MB, D>(shx, shl, pol, b, n, t, v, g0, g1);   \
+  else                                                                  \
+    slab_block_adj<MB, 2 * MB - 1, D>(shx, shl, pol, b, n, t, v, g0, g1);
+      if (mb == 1) {
+        SFAMD_SLAB_BLOCK_ADJ(1)
+      } else if constexpr (D > 8) {
+        if (mb == 2) {
+          SFAMD_SLAB_BLOCK_ADJ(2)
+        } else if constexpr (D > 16) {
+          SFAMD_SLAB_BLOCK_ADJ(3)
+        }
       }
+#undef SFAMD_SLAB_BLOCK_ADJ
     }
     __syncthreads();
     tile_store(shl, li, t, v, D);
