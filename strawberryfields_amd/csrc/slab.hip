@@ -726,6 +726,15 @@ bool plan_tile(Tile& t, int64_t N, int D, int64_t S_out, int64_t Sin, int64_t li
 // read from the gate table in global memory (L1/L2-resident); staging it in LDS
 // cost more occupancy than it saved.
 constexpr int64_t kMfmaTile = 8192;
+// Forward tiles are halved (32 KB: more workgroups per CU to overlap one
+// tile's copies with another's products) when the half still holds 16 whole
+// slabs, i.e. a full 16-fiber chain per block: the ket beamsplitter on the
+// innermost pair at cutoff 16 went from 38% to 42% of the HBM peak, while
+// halving the cutoff-20 DM tiles (which splits their slabs) was slower (DM
+// loss on the innermost mode 38% -> 28%).
+inline int64_t mfma_fwd_tile(int D, int64_t S_out) {
+  return 16 * (int64_t)D * S_out <= kMfmaTile / 2 ? kMfmaTile / 2 : kMfmaTile;
+}
 bool plan_tile_mfma(Tile& t, int64_t N, int D, int64_t S_out, int64_t Sin, int64_t limit) {
   if (!plan_tile(t, N, D, S_out, Sin, limit)) return false;
   if (t.G >= 16) {
@@ -862,7 +871,8 @@ static int bs_slab_mfma(const void* in, void* out, int64_t N, int D, int64_t sp,
                         int conj, int diff, hipStream_t s) {
   Tile t;
   const int p_outer = sp > sq;
-  if (!plan_tile_mfma(t, N, D, p_outer ? sp : sq, p_outer ? sq : sp, kMfmaTile)) return SFAMD_E_ARG;
+  if (!plan_tile_mfma(t, N, D, p_outer ? sp : sq, p_outer ? sq : sp, mfma_fwd_tile(D, p_outer ? sp : sq)))
+    return SFAMD_E_ARG;
   switch (D) {
 #define CASE(d)                                                     \
   case d: {                                                         \
@@ -898,7 +908,7 @@ static int bs_slab_mfma_adj(const void* psi, const void* lo, void* li, int64_t N
 static int loss_slab_mfma(const void* in, void* out, int64_t N, int D, int64_t sb, int64_t sk, const void* coef,
                           hipStream_t s) {
   Tile t;
-  if (!plan_tile_mfma(t, N, D, sb, sk, kMfmaTile)) return SFAMD_E_ARG;
+  if (!plan_tile_mfma(t, N, D, sb, sk, mfma_fwd_tile(D, sb))) return SFAMD_E_ARG;
   switch (D) {
 #define CASE(d)                                             \
   case d: {                                                 \
