@@ -93,7 +93,8 @@ __device__ __forceinline__ void apply_op(const float (&A)[MB][Blk<MB>::NS], cons
 // together and the A operand of each MFMA is one conflict-free ds_read.  A
 // thread computes its entries in batches of kOpBatch (every entry of a batch
 // is read, the element index clamped to the last) before storing them, so
-// that their loads are in flight together.
+// that their loads are in flight together (8 measured slower in the
+// beamsplitter adjoint at cutoff 20: 1148 -> 1413 us).
 constexpr int kOpBatch = 4;
 template <int MB, typename F>
 __device__ __forceinline__ void build_op_lds(F entry, float* dst, int P) {
