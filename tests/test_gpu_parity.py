@@ -189,12 +189,13 @@ def test_beamsplitter_all_mode_pairs(dtype, pure, n, D):
             assert np.abs(got - ref).max() < TOL[dtype], (m1, m2, p)
 
 
-@pytest.mark.parametrize("D", [4, 9, 12])
+@pytest.mark.parametrize("D", [4, 9, 12, 17])
 def test_dm_two_mode_gates_in_one_pass_match_oracle(D):
     """A two-mode gate on a DM whose pair is followed by another mode runs in
     one pass (rho' = G rho G^H on matrix cores): every gate of the beamsplitter
-    family, both mode orders, block sizes of one and two 16-row blocks, against
-    the complex128 oracle's two passes."""
+    family, both mode orders, blocks of one, two and three 16-row blocks (the
+    last at D = 17, a 24M-entry DM), against the complex128 oracle's two
+    passes."""
     n = 3
     init = rand_state(n, D, False, seed=300 + D)
     for kind, p in [("BS", (0.7, 0.9)), ("BS", (-1.1, 2.4)), ("MZ", (1.1, 0.4)), ("S2", (0.3, 1.2)), ("CK", (0.6,))]:
