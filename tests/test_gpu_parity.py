@@ -696,6 +696,36 @@ def test_per_fiber_kernels_match_oracle_and_fd(dtype, pure, n, D):
     assert rel < (1e-8 if dtype == C128 else 1e-5), rel
 
 
+@pytest.mark.parametrize("D", [16, 20])
+def test_dense_gates_on_the_innermost_axis_match_oracle_and_fd(D):
+    """Complex64 dense gates on the last mode of a ket whose fiber count is a
+    multiple of 16 take the staged innermost-axis variant of the matrix-core
+    kernels (each 16-fiber chain copied through LDS): forward against the
+    complex128 oracle, gradient of a photon-number cost against central
+    differences of the oracle."""
+    n = 3
+    rng = np.random.default_rng(500 + D)
+    ops = [("S", (2,), (0.3, 0.7)), ("D", (2,), (0.4, -1.1)), ("BS", (1, 2), (0.6, 0.2)),
+           ("D", (2,), (-0.2, 0.5)), ("S", (2,), (-0.25, 2.0))]
+    ref, _ = fo.run_ops(ops, n, D, pure=True)
+    cc = runtime.CompiledCircuit([(o[0], o[1]) for o in ops], n, D, False, C64, DEV, fuse=0)
+    rows = np.zeros((len(ops), 2))
+    for i, (_, _, p) in enumerate(ops):
+        rows[i, : len(p)] = p
+    params = torch.tensor(rows, dtype=torch.float64, device=DEV, requires_grad=True)
+    st0 = torch.as_tensor(fo.vacuum(n, D, pure=True), dtype=C64, device=DEV)
+    out = runtime.CircuitFunction.apply(params, st0, cc, None)
+    assert np.abs(to_np(out) - ref).max() < TOL[C64]
+    idx = np.indices([D] * n).sum(axis=0)
+    weights = (idx - 1.0) ** 2
+    (runtime.all_fock_probs(out, n, D, False).to(torch.float64) * torch.as_tensor(weights, device=DEV)).sum().backward()
+    g = params.grad.cpu().numpy()
+    gflat = np.array([g[i, j] for i, (_, _, p) in enumerate(ops) for j in range(len(p))])
+    g_fd = _fd_subset(ops, n, D, True, weights, list(range(len(gflat))))
+    rel = np.linalg.norm(gflat - g_fd) / np.linalg.norm(g_fd)
+    assert rel < 1e-5, rel
+
+
 # ------------------------------------------------------------------ batched circuits
 def test_batch_evaluator_matches_per_circuit_autograd():
     rng = np.random.default_rng(61)
